@@ -1,0 +1,220 @@
+"""Benchmark of the SEM operator-apply-and-solve path on one MI355X.
+
+Headline workload: 256x256 quadrilateral elements, N=7 (8x8 GLL nodes per
+element), deformed mesh x = r + 0.1 sin(pi r) sin(pi s), y = s + 0.1 sin(pi r)
+sin(pi s), Helmholtz (-Lap + 1) u with homogeneous Dirichlet data, FP64. A
+step is one operator apply y = M .* QQ'(A u + B u) over the whole mesh
+(4,194,304 local / 3,214,849 global nodes). ``value`` is global DOF/s
+(unique nodes x applies / s). The same line carries the roofline of the
+apply, the FP64 Jacobi-PCG time to 1e-10 relative residual, the mixed
+precision solve, and the CPU baseline (the oracle's reference-faithful apply
+with the dense QQ' factors of mesh.jl:81-82, timed on a bounded sample).
+
+usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-pcg]
+                       [--no-mixed] [--no-cpu] [--rotate 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spectralelements.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "A·u+QQ^T GDOF/s & PCG time-to-1e-10 residual, FP64 vs mixed, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def global_normal_field(se, m, seed):
+    """f ~ N(0,1) i.i.d. per global node (column-major global order,
+    mesh.jl:87-88), scattered to every local copy so it is continuous."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.from_numpy(rng.standard_normal(m.nglobal)).cuda()
+    l2g = m.l2g.long()
+    f = se.empty_field(*m.shape)
+    f.copy_(g[l2g])
+    return f
+
+
+def bench_apply(se, args):
+    import torch
+    n, E = 8, 256
+    sets = []
+    for r in range(args.rotate):
+        m = se.Mesh(n, n, E, E, (False, False), se.deform_sinsin(0.1))
+        M = se.generateMask("DDDD", m)
+        op = se.SEMOperator(1.0, 1.0, m, M)
+        u = global_normal_field(se, m, 2 + r)
+        y = se.empty_field(*m.shape)
+        sets.append((m, op, u, y))
+    m = sets[0][0]
+    st = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        _, op, u, y = sets[i % len(sets)]
+        op(u, out=y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for i in range(args.steps):
+        _, op, u, y = sets[i % len(sets)]
+        op(u, out=y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    # same loop on one buffer set (working set 201 MB, can sit in the 256 MB MALL)
+    _, op, u, y = sets[0]
+    e0.record(st)
+    for i in range(args.steps):
+        op(u, out=y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_one = e0.elapsed_time(e1)
+    return m, ms / args.steps, ms_one / args.steps, sets
+
+
+def bench_pcg(se, m, sets):
+    import torch
+    _, op, _, _ = sets[0]
+    f = global_normal_field(se, m, 2)
+    M = op.M
+    b = se.gatherScatter(se.mask(se.mass(f, m), M), m)
+    tol = 1e-10 * float(b.abs().max())
+    d = op.jacobi()
+    info = se.PCGInfo()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = se.pcg(b, op, opM=se.Diagonal(d), mult=m.mult, tol=tol, info=info)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    r = b - op(x)
+    true_res = float(r.abs().max()) / float(b.abs().max())
+    return {"iters": info.iters, "time_s": wall, "device_ms": info.elapsed_ms,
+            "ms_per_iter": info.elapsed_ms / max(info.iters, 1),
+            "rel_res_recursive": info.res / float(b.abs().max()),
+            "rel_res_true": true_res, "converged": info.converged,
+            "precond": "jacobi", "tol_rel": 1e-10}
+
+
+def bench_mixed(se):
+    """128x128 elements, N=15 (16x16 GLL blocks), deformed, Helmholtz: FP64
+    Jacobi-PCG against FP64 defect correction around an FP32 inner PCG."""
+    import torch
+    m = se.Mesh(16, 16, 128, 128, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    f = global_normal_field(se, m, 3)
+    b = se.gatherScatter(se.mask(se.mass(f, m), M), m)
+    bmax = float(b.abs().max())
+    tol = 1e-10 * bmax
+    d = op64.jacobi()
+    out = {"workload": "128x128 elements, N=15, deformed Helmholtz, f~N(0,1) seed 3"}
+    info = se.PCGInfo()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64 = se.pcg(b, op64, opM=se.Diagonal(d), mult=m.mult, tol=tol, info=info)
+    torch.cuda.synchronize()
+    out["fp64"] = {"iters": info.iters, "time_s": time.perf_counter() - t0,
+                   "rel_res_true": float((b - op64(x64)).abs().max()) / bmax}
+    for prec in ("f32",):
+        op_lo = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+        info = se.PCGInfo()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = se.pcg_mixed(b, op64, op_lo, d=d, tol=tol, inner_tol=1e-5, info=info)
+        torch.cuda.synchronize()
+        out[prec] = {"outer": info.outer, "inner_iters": info.iters,
+                     "time_s": time.perf_counter() - t0, "converged": info.converged,
+                     "rel_res_true": float((b - op64(x)).abs().max()) / bmax,
+                     "rel_err_vs_fp64": float((x - x64).abs().max()) / float(x64.abs().max())}
+    return out
+
+
+def cpu_baseline(budget_s=15.0):
+    """The oracle's reference-faithful apply (ABu reshape-GEMMs and the dense
+    QQtx * u * QQty' of gatherScatter.jl:13) at the headline size."""
+    import numpy as np
+    from oracle import sem_oracle as o
+    mo = o.Mesh(8, 8, 256, 256, (False, False), o.deform_sinsin(0.1), dense_gs=True)
+    Mo = o.generateMask("DDDD", mo)
+    u = np.random.default_rng(2).standard_normal(mo.x.shape)
+    o.opLHS(u, 1.0, 1.0, mo, Mo)
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or (time.perf_counter() - t0 < budget_s and reps < 200):
+        o.opLHS(u, 1.0, 1.0, mo, Mo)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    try:
+        import threadpoolctl
+        th = max(i.get("num_threads", 1) for i in threadpoolctl.threadpool_info())
+    except Exception:
+        th = os.cpu_count()
+    return {"value": mo.nglobal / dt / 1e9, "unit": "GDOF/s", "cores": th, "kind": "port",
+            "sample": f"{reps} applies of the reference-faithful NumPy/OpenBLAS apply "
+                      f"(dense QQ' GEMMs) at 256x256 N=7, {dt * 1e3:.1f} ms each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rotate", type=int, default=2)
+    ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the operator would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import semhip as se
+    torch.cuda.set_device(0)
+    log("apply")
+    m, ms, ms_one, sets = bench_apply(se, args)
+    nl, ng = m.nlocal, m.nglobal
+    alg_bytes = 48 * nl
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": ng / (ms * 1e-3) / 1e9, "unit": "GDOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: u ~ N(0,1) i.i.d. per global node (seeds 2..), scattered to local copies",
+        "config": {"workload": "FP64 Helmholtz apply y = M.*QQ'(A u + B u), 256x256 elements, "
+                               "N=7, deformed sin-sin mesh, Dirichlet on all sides",
+                   "elements": [256, 256], "N": 7, "n_local": nl, "n_global": ng,
+                   "buffer_sets_rotated": args.rotate},
+        "gdofs_local": nl / (ms * 1e-3) / 1e9,
+        "ms_per_step_single_set": ms_one,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused apply (k_ax strip kernel + k_seams), 48 B per local node"},
+    }
+    if not args.no_pcg:
+        log("pcg")
+        res["pcg_fp64"] = bench_pcg(se, m, sets)
+        it = res["pcg_fp64"]
+        res["pcg_fp64"]["achieved_GBps_120B"] = 120 * nl / (it["ms_per_iter"] * 1e-3) / 1e9
+    del sets
+    torch.cuda.empty_cache()
+    if not args.no_mixed:
+        log("mixed")
+        res["mixed"] = bench_mixed(se)
+    if not args.no_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

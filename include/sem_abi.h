@@ -1,0 +1,243 @@
+/*
+ * sem_abi.h -- C ABI of libsemhip.so, the MI355X (gfx950) operator-apply-and-
+ * solve path of SpectralElements.jl.
+ *
+ * Conventions
+ *  - Every nodal field is a device array of n_local = (nr*Ex)*(ns*Ey) values in
+ *    the reference's local layout: column-major (nr*Ex, ns*Ey), element
+ *    (ex,ey) owning rows ex*nr..ex*nr+nr-1 and columns ey*ns..ey*ns+ns-1
+ *    (mesh.jl:94-100). Julia arrays and Fortran-ordered NumPy/torch views pass
+ *    through unchanged.
+ *  - Small dense matrices (D, interpolation, ABu operands) are column-major.
+ *  - Field pointers are device pointers owned by the caller and borrowed for
+ *    the duration of the call; host pointers are named *_host.
+ *  - All calls are ordered on the context's HIP stream. Calls that return host
+ *    scalars synchronise that stream.
+ *  - Return value: SEM_OK (0), SEM_WARN_MAXITER (1, the reference's
+ *    warn-and-return of pcg.jl:39) or a negative error code; the message of
+ *    the last error on the calling thread is sem_last_error().
+ */
+#ifndef SEM_ABI_H
+#define SEM_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEM_ABI_VERSION 1
+
+enum sem_status {
+  SEM_OK = 0,
+  SEM_WARN_MAXITER = 1,
+  SEM_ERR_ARG = -1,
+  SEM_ERR_HIP = -2,
+  SEM_ERR_UNSUPPORTED = -3,
+  SEM_ERR_NONFINITE = -4,
+  SEM_ERR_BREAKDOWN = -5
+};
+
+enum sem_precision { SEM_F64 = 0, SEM_F32 = 1, SEM_F16_MFMA = 2, SEM_BF16_MFMA = 3 };
+
+enum sem_deform {
+  SEM_DEFORM_NONE = 0,     /* fixU, mesh.jl:6-8 */
+  SEM_DEFORM_SINSIN = 1,   /* x += a sin(pi x) sin(pi y), y += a sin(pi x) sin(pi y) */
+  SEM_DEFORM_AFFINE01 = 2  /* usrgeom of p2d_explicit.jl:78: [-1,1]^2 -> [0,1]^2 */
+};
+
+/* mesh field selectors for sem_mesh_field (names of struct Mesh, mesh.jl:25-64) */
+enum sem_mesh_field_id {
+  SEM_X = 0, SEM_Y, SEM_JAC, SEM_JACI, SEM_RX, SEM_RY, SEM_SX, SEM_SY,
+  SEM_B, SEM_BI, SEM_G11, SEM_G12, SEM_G22, SEM_MULT, SEM_NUM_MESH_FIELDS
+};
+
+/* Dirichlet mask bits (generateMask, mesh.jl:149-175: bc = [xmin,xmax,ymin,ymax]) */
+enum sem_bc_bits { SEM_BC_XMIN = 1, SEM_BC_XMAX = 2, SEM_BC_YMIN = 4, SEM_BC_YMAX = 8 };
+
+enum sem_precond {
+  SEM_PRECOND_IDENTITY = 0, /* opPrecond of Diffusion, diffusion.jl:47-49 */
+  SEM_PRECOND_DIAG = 1      /* z = d .* r with a caller/operator diagonal d: the
+                               Jacobi inverse (sem_operator_jacobi) or
+                               1 ./ (B * b0) of convectionDiffusion.jl:87-91 */
+};
+
+typedef struct sem_ctx sem_ctx;
+typedef struct sem_mesh sem_mesh;
+typedef struct sem_operator sem_operator;
+typedef struct sem_gsmap sem_gsmap;
+
+/* ---------------------------------------------------------------- context */
+int sem_abi_version(void);
+const char* sem_last_error(void);
+/* stream: a hipStream_t (NULL = the legacy default stream) */
+int sem_ctx_create(int device, void* stream, sem_ctx** out);
+int sem_ctx_destroy(sem_ctx* ctx);
+int sem_ctx_set_stream(sem_ctx* ctx, void* stream);
+int sem_ctx_synchronize(sem_ctx* ctx);
+
+/* ------------------------------------------------ L0 host primitives (FP64) */
+/* FastGaussQuadrature.gausslobatto(n) as called at mesh.jl:70-71, semmesh.jl:11 */
+int sem_gausslobatto(int n, double* z_host, double* w_host);
+/* derivMat(x), derivMat.jl:9-35; D_host is n*n column-major */
+int sem_derivmat(int n, const double* x_host, double* D_host);
+/* interpMat(xo, xi), interp.jl:10-35; J_host is no*ni column-major */
+int sem_interpmat(int no, const double* xo_host, int ni, const double* xi_host, double* J_host);
+/* semmesh(E, n), semmesh.jl:9-27: z_host, w_host have E*n entries */
+int sem_semmesh(int E, int n, double* z_host, double* w_host);
+
+/* ------------------------------------------------------------------- mesh */
+/* Mesh(nr, ns, Ex, Ey, ifperiodic, deform), mesh.jl:66-133. GLL nodes and D
+ * on the host; coordinates, the deformation, jac (jac.jl:24-40), B, Bi, G11,
+ * G12, G22 and mult are computed in FP64 on the GPU. */
+int sem_mesh_create(sem_ctx* ctx, int nr, int ns, int Ex, int Ey, int periodic_x,
+                    int periodic_y, int deform_kind, double deform_param, sem_mesh** out);
+/* Replaces the coordinates by caller-deformed ones (a user deform callback,
+ * mesh.jl:108) and recomputes the geometric factors on the GPU. */
+int sem_mesh_set_coords(sem_ctx* ctx, sem_mesh* mesh, const double* x, const double* y);
+int sem_mesh_destroy(sem_mesh* mesh);
+/* sizes: nr, ns, Ex, Ey, n_local, n_global (any pointer may be NULL) */
+int sem_mesh_sizes(const sem_mesh* mesh, int* nr, int* ns, int* Ex, int* Ey,
+                   int64_t* n_local, int64_t* n_global);
+/* device pointer to a mesh array (read-only, n_local doubles) */
+int sem_mesh_field(const sem_mesh* mesh, int which, const double** dev_ptr);
+/* device pointers to Dr, Ds (nr*nr, ns*ns column-major) and the GLL nodes
+ * and weights zr, wr, zs, ws */
+int sem_mesh_dmat(const sem_mesh* mesh, const double** Dr, const double** Ds);
+int sem_mesh_gll(const sem_mesh* mesh, const double** zr, const double** wr,
+                 const double** zs, const double** ws);
+/* local-to-global numbering (int32 device array, 0-based, column-major
+ * global grid): the local l2g intended by mesh.jl:87-89 */
+int sem_mesh_l2g(const sem_mesh* mesh, const int32_t** dev_l2g);
+/* generateMask(bc, msh), mesh.jl:149-175: bc = 4 chars 'D'/'N' for
+ * [xmin,xmax,ymin,ymax]; returns SEM_BC_* bits (periodic directions cleared) */
+int sem_mask_bits(const sem_mesh* mesh, const char* bc4, int* bits);
+/* materialises the mask M (float64 0/1 field) for the given bits */
+int sem_mask_fill(sem_ctx* ctx, const sem_mesh* mesh, int bits, double* M);
+
+/* ----------------------------------------------------- gather-scatter map */
+/* Deterministic QQ' over a precomputed map: local indices sorted by global
+ * id (stable, so copies of a node are in local-index order), segment offsets.
+ * Built from any 0-based local->global table (the NNlib form of
+ * gatherScatter.jl:27-34 takes such a table). */
+int sem_gsmap_create(sem_ctx* ctx, int64_t n_local, const int32_t* l2g_host,
+                     int64_t n_global, sem_gsmap** out);
+int sem_gsmap_destroy(sem_gsmap* map);
+int sem_gsmap_apply(sem_ctx* ctx, const sem_gsmap* map, const double* u, double* y);
+/* gatherScatter(u, msh), gatherScatter.jl:18-21 (dense QQtx*u*QQty' in the
+ * reference; value-identical here, x-pairs summed before y-pairs) */
+int sem_gather_scatter(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);
+int sem_gather_scatter_f32(sem_ctx* ctx, const sem_mesh* mesh, const float* u, float* y);
+
+/* ---------------------------------------------------- pointwise and ABu */
+/* mask(u, M) = M .* u, mask.jl:10-18 (M == NULL: copy) */
+int sem_mask(sem_ctx* ctx, int64_t n, const double* M, const double* u, double* y);
+/* mask by bits (structured M = mx my') */
+int sem_mask_bc(sem_ctx* ctx, const sem_mesh* mesh, int bits, const double* u, double* y);
+/* mass(u, msh) = B .* u, mass.jl:12-22 */
+int sem_mass(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);
+/* ABu(As, Br, u), ABu.jl:9-37, u is m x n column-major. As (ma x na) or Br
+ * (mb x nb) may be NULL for the reference's []. The output is
+ * (m*mb/nb) x (n*ma/na). The pullback ABu(As', Br', dv) of ABu.jl:39-41 is
+ * ABu with trans = 1 (both operands transposed). */
+int sem_abu(sem_ctx* ctx, int m, int n, const double* u, const double* As, int ma,
+            int na, const double* Br, int mb, int nb, int trans, double* y);
+
+/* ------------------------------------------------------ element operators */
+/* lapl(u, msh), lapl.jl:26-36 (element-local weak Laplacian, no QQ') */
+int sem_lapl(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);
+/* hlmz(u, nu, k, msh) = nu .* lapl(u) + k .* mass(u), hlmz.jl:12-19; nu_f or
+ * k_f (device fields) replace the scalars when non-NULL */
+int sem_hlmz(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double nu,
+             const double* nu_f, double k, const double* k_f, double* y);
+
+/* ------------------------------------------------------- fused operator */
+typedef struct sem_op_desc {
+  double nu;             /* scalar nu (used when nu_f == NULL) */
+  double k;              /* scalar mass coefficient (bdfB[1] in diffusion.jl:40) */
+  const double* nu_f;    /* optional nu field, post-multiplies A u (hlmz.jl:15) */
+  const double* k_f;     /* optional k field */
+  const double* G11;     /* optional geometric-factor overrides (the 12-argument */
+  const double* G12;     /*   lapl of lapl.jl:54-68 takes G11/G12/G22 from the   */
+  const double* G22;     /*   caller, e.g. nu folded in: nu .* G, semPS.jl:161)  */
+  const double* B;       /* optional mass override */
+  int bc_bits;           /* Dirichlet mask bits (0 = no mask) */
+  int assemble;          /* 1: y = M .* QQ'(nu A u + k B u) (opLHS, diffusion.jl:36-45);
+                            0: element-local nu A u + k B u (hlmz.jl:12) */
+} sem_op_desc;
+
+/* Builds a device operator. precision SEM_F64 keeps FP64 factors; SEM_F32
+ * stores them in FP32; SEM_F16_MFMA / SEM_BF16_MFMA (nr == ns == 16) run the
+ * Dr/Ds contractions as FP16/BF16-input, FP32-accumulate MFMA with FP32
+ * factors and fields. */
+int sem_operator_create(sem_ctx* ctx, const sem_mesh* mesh, const sem_op_desc* desc,
+                        int precision, sem_operator** out);
+int sem_operator_destroy(sem_operator* op);
+int sem_operator_precision(const sem_operator* op, int* precision);
+/* y = op(u); u and y are double* for SEM_F64 and float* otherwise */
+int sem_operator_apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y);
+/* Jacobi inverse diagonal dinv = M .* 1 ./ QQ'(diag_local), FP64 */
+int sem_operator_jacobi(sem_ctx* ctx, const sem_operator* op, double* dinv);
+
+/* --------------------------------------------------------------- solvers */
+typedef struct sem_pcg_info {
+  int iters;        /* iterations performed (k of pcg.jl) */
+  int converged;    /* 1 if norm(r, Inf) <= tol */
+  double res;       /* final norm(r, Inf) of the recursive residual */
+  double elapsed_ms;/* device time of the solve */
+  int outer;        /* defect-correction outer iterations (mixed solver) */
+  int inner_total;  /* total inner iterations (mixed solver) */
+} sem_pcg_info;
+
+/* pcg!(x, b, opA; opM, mult, tol, maxiter), pcg.jl:16-79, fused on the GPU:
+ * x0 = 0 (pcg.jl:25; pcg! ignores the incoming x, :71), stop when
+ * norm(r, Inf) <= tol (absolute, :36), mult-weighted inner products with the
+ * operator mesh's mult (:45, :52), SEM_WARN_MAXITER at k == maxiter (:39).
+ * precond: SEM_PRECOND_IDENTITY or SEM_PRECOND_DIAG with the device diagonal
+ * d (z = d .* r). For SEM_F32 operators b, x and d are float*. */
+int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
+            const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
+
+/* Mixed precision: FP64 defect correction around a low-precision inner PCG.
+ * Outer loop in FP64: r = b - op64(x) (true residual), inner: op_lo d = r
+ * solved by PCG to relative tolerance inner_tol in FP32, x += d, until
+ * norm(r, Inf) <= tol. If plain defect correction stops contracting, the
+ * remaining iterations run as FP64 flexible CG preconditioned by the inner
+ * solve (mode recorded in info->outer). d64 is the FP64 Jacobi diagonal
+ * (or NULL for identity). */
+int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
+                  const double* d64, const double* b, double* x, double tol,
+                  double inner_tol, int max_outer, int max_inner, sem_pcg_info* info);
+
+/* ----------------------------------------------------- vector utilities */
+/* sum(a .* b .* mult) with the mesh's mult (pcg.jl:45); result to host */
+int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* b,
+             double* result_host);
+/* norm(a, Inf); result to host */
+int sem_norm_inf(sem_ctx* ctx, int64_t n, const double* a, double* result_host);
+
+/* ------------------------------------------------------ adjoint kernels */
+/* grad(u, msh), grad.jl:15-34: ux = rx ur + sx us, uy = ry ur + sy us */
+int sem_grad(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* ux, double* uy);
+/* adjoint of grad as a map L -> 2L: Dr'(rx vx + ry vy) + (sx vx + sy vy) Ds
+ * (the transpose of grad.jl:15-34; gradT of grad.jl:44-63 split per input) */
+int sem_grad_t(sem_ctx* ctx, const sem_mesh* mesh, const double* vx, const double* vy,
+               double* y);
+/* d/dnu of lambda' A(nu) u for A(nu) = lapl with G_ij <- nu .* G_ij
+ * (semPS.jl:161-165): g = B .* (grad(lambda) . grad(u)) per local node, the
+ * per-element kernel of the adjoint gradient (adjoint.jl:13-17) */
+int sem_adjoint_dnu(sem_ctx* ctx, const sem_mesh* mesh, const double* u,
+                    const double* lam, double* g);
+/* advect(T, vx, vy, mshV, mshD, Jr, Js), advect.jl:45-64: dealiased
+ * Js'Jr'[B_D (J vx . J grad T)] with Jr = interpMat(zr_D, zr_V) */
+int sem_advect(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD,
+               const double* T, const double* vx, const double* vy, double* y);
+/* transpose of advect with respect to T (for the discrete adjoint) */
+int sem_advect_t(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD,
+                 const double* w, const double* vx, const double* vy, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEM_ABI_H */

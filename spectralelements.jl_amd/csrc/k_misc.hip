@@ -1,0 +1,667 @@
+// Setup and utility kernels: coordinates and FP64 geometry (mesh.jl:98-123,
+// jac.jl:24-40), mult, masks, the map-based gather-scatter, ABu, reductions,
+// grad / grad' and the adjoint dJ/dnu kernel, dealiased advection.
+#include "sem_common.h"
+
+namespace sem {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__global__ void k_coords(double* __restrict__ x, double* __restrict__ y,
+                         const double* __restrict__ xe, const double* __restrict__ ye, int L,
+                         long n, int deform, double amp) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  double X = xe[i], Y = ye[j];
+  if (deform == SEM_DEFORM_SINSIN) {
+    const double s = amp * sin(kPi * X) * sin(kPi * Y);
+    x[t] = X + s;
+    y[t] = Y + s;
+  } else if (deform == SEM_DEFORM_AFFINE01) {
+    x[t] = 0.5 * (X + 1.0);
+    y[t] = 0.5 * (Y + 1.0);
+  } else {
+    x[t] = X;
+    y[t] = Y;
+  }
+}
+
+// One thread per local node: xr = Dr x (within the element column), xs = x Ds',
+// then J, Ji, rx, ry, sx, sy, B = J (wr ws), Bi, G11, G12, G22 as mesh.jl:114-123.
+__global__ void k_geometry(const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ Dr, const double* __restrict__ Ds,
+                           const double* __restrict__ wr, const double* __restrict__ ws, int nr,
+                           int ns, int L, long n, double* Jac, double* Jaci, double* rx,
+                           double* ry, double* sx, double* sy, double* B, double* Bi,
+                           double* G11, double* G12, double* G22) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  const int a = i % nr, i0 = i - a;
+  const int b = j % ns, j0 = j - b;
+  double xr = 0, yr = 0, xs = 0, ys = 0;
+  for (int k = 0; k < nr; ++k) {
+    const double d = Dr[a + nr * k];
+    xr += d * x[(i0 + k) + (long)L * j];
+    yr += d * y[(i0 + k) + (long)L * j];
+  }
+  for (int k = 0; k < ns; ++k) {
+    const double d = Ds[b + ns * k];
+    xs += x[i + (long)L * (j0 + k)] * d;
+    ys += y[i + (long)L * (j0 + k)] * d;
+  }
+  const double J = xr * ys - xs * yr;
+  const double Ji = 1.0 / J;
+  const double vrx = Ji * ys, vry = -Ji * xs, vsx = -Ji * yr, vsy = Ji * xr;
+  const double w = wr[a] * ws[b];
+  const double vB = J * w;
+  Jac[t] = J;
+  Jaci[t] = Ji;
+  rx[t] = vrx;
+  ry[t] = vry;
+  sx[t] = vsx;
+  sy[t] = vsy;
+  B[t] = vB;
+  Bi[t] = 1.0 / vB;
+  G11[t] = vB * (vrx * vrx + vry * vry);
+  G12[t] = vB * (vrx * vsx + vry * vsy);
+  G22[t] = vB * (vsx * vsx + vsy * vsy);
+}
+
+__device__ __forceinline__ double mult1(int i, int n, int L, int per) {
+  // 1/(multiplicity along one direction) of local row i
+  const int a = i % n;
+  if (a == 0 && (i > 0 || per)) return 0.5;
+  if (a == n - 1 && (i < L - 1 || per)) return 0.5;
+  return 1.0;
+}
+
+__global__ void k_mult(double* __restrict__ mult, int nr, int ns, int L, int NC, int per_x,
+                       int per_y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  // n == 1 point per element cannot occur (nr, ns >= 2)
+  mult[t] = mult1(i, nr, L, per_x) * mult1(j, ns, NC, per_y);
+}
+
+__device__ __forceinline__ bool mask_on(int i, int j, int L, int NC, int bc) {
+  if ((bc & SEM_BC_XMIN) && i == 0) return false;
+  if ((bc & SEM_BC_XMAX) && i == L - 1) return false;
+  if ((bc & SEM_BC_YMIN) && j == 0) return false;
+  if ((bc & SEM_BC_YMAX) && j == NC - 1) return false;
+  return true;
+}
+
+__global__ void k_mask_fill(double* __restrict__ M, int L, int NC, int bc, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  M[t] = mask_on(i, j, L, NC, bc) ? 1.0 : 0.0;
+}
+
+__global__ void k_mask_bc(const double* __restrict__ u, double* __restrict__ y, int L, int NC,
+                          int bc, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  y[t] = (mask_on(i, j, L, NC, bc) ? 1.0 : 0.0) * u[t];
+}
+
+__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ u,
+                      double* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  y[t] = a ? a[t] * u[t] : u[t];
+}
+
+// Map-based QQ': one thread per global node, copies summed pairwise in map
+// order ((v0+v1)+(v2+v3)), the sum written back to every copy.
+template <typename T>
+__global__ void k_gsmap(const int32_t* __restrict__ perm, const int32_t* __restrict__ seg,
+                        const T* __restrict__ u, T* __restrict__ y, long ng) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int s0 = seg[g], s1 = seg[g + 1];
+  const int n = s1 - s0;
+  T s;
+  if (n == 1) {
+    s = u[perm[s0]];
+  } else if (n == 2) {
+    s = u[perm[s0]] + u[perm[s0 + 1]];
+  } else if (n == 4) {
+    s = (u[perm[s0]] + u[perm[s0 + 1]]) + (u[perm[s0 + 2]] + u[perm[s0 + 3]]);
+  } else {
+    // general segment: pairwise sums of consecutive pairs, left to right
+    s = T(0);
+    int k = s0;
+    bool first = true;
+    for (; k + 1 < s1; k += 2) {
+      const T p = u[perm[k]] + u[perm[k + 1]];
+      s = first ? p : s + p;
+      first = false;
+    }
+    if (k < s1) s = first ? u[perm[k]] : s + u[perm[k]];
+  }
+  for (int k = s0; k < s1; ++k) y[perm[k]] = s;
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (float)x[t];
+}
+__global__ void k_f32_to_f64(const float* __restrict__ x, double* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (double)x[t];
+}
+
+// ABu, ABu.jl:9-37. Stage 1 (Br on groups of nb rows): tmp (m1 x n);
+// stage 2 (As on blocks of na columns): y (m1 x n2). trans: operands transposed.
+__global__ void k_abu_r(const double* __restrict__ u, const double* __restrict__ Br, int m,
+                        int n, int mb, int nb, int trans, double* __restrict__ out) {
+  const int m1 = (m / nb) * mb;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m1 * n) return;
+  const int r = (int)(t % m1);
+  const int c = (int)(t / m1);
+  const int grp = r / mb, rr = r - grp * mb;
+  double acc = 0;
+  for (int k = 0; k < nb; ++k) {
+    // Br is mb x nb column-major; transposed operand is stored nb x mb
+    const double bv = trans ? Br[k + nb * rr] : Br[rr + mb * k];
+    acc += bv * u[(grp * nb + k) + (long)m * c];
+  }
+  out[t] = acc;
+}
+
+__global__ void k_abu_s(const double* __restrict__ u, const double* __restrict__ As, int m,
+                        int n, int ma, int na, int trans, double* __restrict__ out) {
+  const int n2 = (n / na) * ma;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m * n2) return;
+  const int r = (int)(t % m);
+  const int c = (int)(t / m);
+  const int blk = c / ma, cc = c - blk * ma;
+  double acc = 0;
+  for (int l = 0; l < na; ++l) {
+    const double av = trans ? As[l + na * cc] : As[cc + ma * l];
+    acc += u[r + (long)m * (blk * na + l)] * av;
+  }
+  out[t] = acc;
+}
+
+// ---- reductions (deterministic two-stage: fixed per-block partials, then one
+// block sums them in index order)
+__device__ double block_reduce_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] = sh[tid] + sh[tid + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ double block_reduce_max(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double o = sh[tid + s];
+      // NaN propagates (the reference's norm(NaN) is NaN)
+      sh[tid] = (o > sh[tid] || o != o) ? o : sh[tid];
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256)
+k_wdot_part(const double* __restrict__ a, const double* __restrict__ b, int P, int L, int NC,
+            int per_x, int per_y, long n, double* part) {
+  __shared__ double sh[256];
+  double acc = 0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % L);
+    const int j = (int)(t / L);
+    acc += a[t] * b[t] * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
+  }
+  const double s = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256)
+k_absmax_part(const double* __restrict__ a, long n, double* part) {
+  __shared__ double sh[256];
+  double m = 0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const double v = fabs(a[t]);
+    m = (v > m || v != v) ? v : m;
+  }
+  const double r = block_reduce_max(m, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256) k_final(const double* part, int n, int op, double* out) {
+  __shared__ double sh[256];
+  double acc = 0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const double v = part[t];
+    if (op == 0) acc += v;
+    else acc = (v > acc || v != v) ? v : acc;
+  }
+  const double r = op == 0 ? block_reduce_sum(acc, sh) : block_reduce_max(acc, sh);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+// ---- gradient (grad.jl:15-34), its transpose, and the dJ/dnu kernel
+__global__ void k_grad(const double* __restrict__ u, const double* __restrict__ Dr,
+                       const double* __restrict__ Ds, const double* __restrict__ rx,
+                       const double* __restrict__ ry, const double* __restrict__ sx,
+                       const double* __restrict__ sy, int nr, int ns, int L, long n,
+                       double* __restrict__ ux, double* __restrict__ uy) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  const int a = i % nr, i0 = i - a;
+  const int b = j % ns, j0 = j - b;
+  double ur = 0, us = 0;
+  for (int k = 0; k < nr; ++k) ur += Dr[a + nr * k] * u[(i0 + k) + (long)L * j];
+  for (int k = 0; k < ns; ++k) us += u[i + (long)L * (j0 + k)] * Ds[b + ns * k];
+  ux[t] = rx[t] * ur + sx[t] * us;
+  uy[t] = ry[t] * ur + sy[t] * us;
+}
+
+__global__ void k_grad_t(const double* __restrict__ vx, const double* __restrict__ vy,
+                         const double* __restrict__ Dr, const double* __restrict__ Ds,
+                         const double* __restrict__ rx, const double* __restrict__ ry,
+                         const double* __restrict__ sx, const double* __restrict__ sy, int nr,
+                         int ns, int L, long n, double* __restrict__ y) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  const int a = i % nr, i0 = i - a;
+  const int b = j % ns, j0 = j - b;
+  double acc = 0;
+  // Dr'(rx vx + ry vy): sum over rows k of the element column
+  for (int k = 0; k < nr; ++k) {
+    const long q = (i0 + k) + (long)L * j;
+    acc += Dr[k + nr * a] * (rx[q] * vx[q] + ry[q] * vy[q]);
+  }
+  double acc2 = 0;
+  for (int k = 0; k < ns; ++k) {
+    const long q = i + (long)L * (j0 + k);
+    acc2 += (sx[q] * vx[q] + sy[q] * vy[q]) * Ds[k + ns * b];
+  }
+  y[t] = acc + acc2;
+}
+
+// g = B .* (grad(lambda) . grad(u)): with G_ij <- nu G_ij the element form is
+// lambda' A(nu) u = sum_nodes nu B (grad lambda . grad u), so d/dnu_local is
+// the integrand B (grad lambda . grad u).
+__global__ void k_adjoint_dnu(const double* __restrict__ u, const double* __restrict__ lam,
+                              const double* __restrict__ Dr, const double* __restrict__ Ds,
+                              const double* __restrict__ rx, const double* __restrict__ ry,
+                              const double* __restrict__ sx, const double* __restrict__ sy,
+                              const double* __restrict__ B, int nr, int ns, int L, long n,
+                              double* __restrict__ g) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % L);
+  const int j = (int)(t / L);
+  const int a = i % nr, i0 = i - a;
+  const int b = j % ns, j0 = j - b;
+  double ur = 0, us = 0, lr = 0, ls = 0;
+  for (int k = 0; k < nr; ++k) {
+    const long q = (i0 + k) + (long)L * j;
+    ur += Dr[a + nr * k] * u[q];
+    lr += Dr[a + nr * k] * lam[q];
+  }
+  for (int k = 0; k < ns; ++k) {
+    const long q = i + (long)L * (j0 + k);
+    us += u[q] * Ds[b + ns * k];
+    ls += lam[q] * Ds[b + ns * k];
+  }
+  const double ux = rx[t] * ur + sx[t] * us, uy = ry[t] * ur + sy[t] * us;
+  const double lx = rx[t] * lr + sx[t] * ls, ly = ry[t] * lr + sy[t] * ls;
+  g[t] = B[t] * (lx * ux + ly * uy);
+}
+
+// Dealiased advection, advect.jl:45-64, one workgroup per element, with
+// J = Jr (x) Js the element interpolation to the dealiasing grid:
+//   forward:   y = J' [ B_D ((J vx)(J Tx) + (J vy)(J Ty)) ]
+//   transpose: the same linear map in T transposed factor by factor,
+//              y = grad'( J'[(J vx) B_D (J w)], J'[(J vy) B_D (J w)] )
+// with grad'(qx, qy) = Dr'(rx qx + ry qy) + (sx qx + sy qy) Ds.
+constexpr int kMaxPD = 24;
+
+__global__ void __launch_bounds__(256)
+k_advect(const double* __restrict__ T, const double* __restrict__ vx,
+         const double* __restrict__ vy, const double* __restrict__ Dr,
+         const double* __restrict__ Ds, const double* __restrict__ rx,
+         const double* __restrict__ ry, const double* __restrict__ sx,
+         const double* __restrict__ sy, const double* __restrict__ BD,
+         const double* __restrict__ Jr, const double* __restrict__ Js, int p, int pd, int Ex,
+         int Ey, double* __restrict__ y, int transpose) {
+  // per-element scratch in LDS
+  __shared__ double sT[kMaxP * kMaxP], sA[kMaxP * kMaxP], sB[kMaxP * kMaxP];
+  __shared__ double dA[kMaxPD * kMaxP], dB[kMaxPD * kMaxPD], dC[kMaxPD * kMaxPD],
+      dD[kMaxPD * kMaxPD];
+  const int ex = blockIdx.x % Ex, ey = blockIdx.x / Ex;
+  const int L = p * Ex, LD = pd * Ex;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  auto gi = [&](int a, int b) { return (long)(ex * p + a) + (long)L * (ey * p + b); };
+  auto gd = [&](int a, int b) { return (long)(ex * pd + a) + (long)LD * (ey * pd + b); };
+  if (!transpose) {
+    for (int q = tid; q < p * p; q += nt) sT[q] = T[gi(q % p, q / p)];
+    __syncthreads();
+    // Tx, Ty on the velocity grid
+    for (int q = tid; q < p * p; q += nt) {
+      const int a = q % p, b = q / p;
+      double ur = 0, us = 0;
+      for (int k = 0; k < p; ++k) ur += Dr[a + p * k] * sT[k + p * b];
+      for (int k = 0; k < p; ++k) us += sT[a + p * k] * Ds[b + p * k];
+      const long g = gi(a, b);
+      sA[q] = rx[g] * ur + sx[g] * us;
+      sB[q] = ry[g] * ur + sy[g] * us;
+    }
+    __syncthreads();
+    // interpolate four fields to the dealiasing grid: W = Jr F Js'
+    // field 0: Tx, 1: Ty, 2: vx, 3: vy; accumulate c = Jvx JTx + Jvy JTy
+    for (int q = tid; q < pd * pd; q += nt) dD[q] = 0;
+    __syncthreads();
+    for (int f = 0; f < 4; ++f) {
+      for (int q = tid; q < pd * p; q += nt) {  // rows: dA(ad, b) = sum_k Jr(ad,k) F(k,b)
+        const int ad = q % pd, b = q / pd;
+        double acc = 0;
+        for (int k = 0; k < p; ++k) {
+          double Fv;
+          if (f == 0) Fv = sA[k + p * b];
+          else if (f == 1) Fv = sB[k + p * b];
+          else if (f == 2) Fv = vx[gi(k, b)];
+          else Fv = vy[gi(k, b)];
+          acc += Jr[ad + pd * k] * Fv;
+        }
+        dA[q] = acc;
+      }
+      __syncthreads();
+      for (int q = tid; q < pd * pd; q += nt) {
+        const int ad = q % pd, bd = q / pd;
+        double acc = 0;
+        for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
+        if (f == 0) dB[q] = acc;
+        else if (f == 1) dC[q] = acc;
+        else if (f == 2) dD[q] = acc * dB[q];
+        else dD[q] = (dD[q] + acc * dC[q]) * BD[gd(ad, bd)];
+      }
+      __syncthreads();
+    }
+    // back to the velocity grid: y = Jr' C Js
+    for (int q = tid; q < p * pd; q += nt) {
+      const int a = q % p, bd = q / p;
+      double acc = 0;
+      for (int k = 0; k < pd; ++k) acc += Jr[k + pd * a] * dD[k + pd * bd];
+      dA[q] = acc;  // (p x pd)
+    }
+    __syncthreads();
+    for (int q = tid; q < p * p; q += nt) {
+      const int a = q % p, b = q / p;
+      double acc = 0;
+      for (int k = 0; k < pd; ++k) acc += dA[a + p * k] * Js[k + pd * b];
+      y[gi(a, b)] = acc;
+    }
+  } else {
+    // transpose with respect to T: y = grad'( J'[B_D Jvx (J w)], J'[B_D Jvy (J w)] )
+    for (int q = tid; q < p * p; q += nt) sT[q] = T[gi(q % p, q / p)];  // T holds w
+    __syncthreads();
+    // c = B_D .* (J w)
+    for (int q = tid; q < pd * p; q += nt) {
+      const int ad = q % pd, b = q / pd;
+      double acc = 0;
+      for (int k = 0; k < p; ++k) acc += Jr[ad + pd * k] * sT[k + p * b];
+      dA[q] = acc;
+    }
+    __syncthreads();
+    for (int q = tid; q < pd * pd; q += nt) {
+      const int ad = q % pd, bd = q / pd;
+      double acc = 0;
+      for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
+      dD[q] = acc * BD[gd(ad, bd)];
+    }
+    __syncthreads();
+    // J vx, J vy on the dealiasing grid -> dB, dC, then products with c
+    for (int f = 0; f < 2; ++f) {
+      for (int q = tid; q < pd * p; q += nt) {
+        const int ad = q % pd, b = q / pd;
+        double acc = 0;
+        for (int k = 0; k < p; ++k) acc += Jr[ad + pd * k] * (f == 0 ? vx[gi(k, b)] : vy[gi(k, b)]);
+        dA[q] = acc;
+      }
+      __syncthreads();
+      for (int q = tid; q < pd * pd; q += nt) {
+        const int ad = q % pd, bd = q / pd;
+        double acc = 0;
+        for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
+        if (f == 0) dB[q] = acc * dD[q];
+        else dC[q] = acc * dD[q];
+      }
+      __syncthreads();
+    }
+    // back-interpolate both: sA = J' dB, sB = J' dC
+    for (int f = 0; f < 2; ++f) {
+      const double* src = f == 0 ? dB : dC;
+      for (int q = tid; q < p * pd; q += nt) {
+        const int a = q % p, bd = q / p;
+        double acc = 0;
+        for (int k = 0; k < pd; ++k) acc += Jr[k + pd * a] * src[k + pd * bd];
+        dA[q] = acc;
+      }
+      __syncthreads();
+      for (int q = tid; q < p * p; q += nt) {
+        const int a = q % p, b = q / p;
+        double acc = 0;
+        for (int k = 0; k < pd; ++k) acc += dA[a + p * k] * Js[k + pd * b];
+        (f == 0 ? sA : sB)[q] = acc;
+      }
+      __syncthreads();
+    }
+    // grad'(sA, sB) = Dr'(rx sA + ry sB) + (sx sA + sy sB) Ds
+    for (int q = tid; q < p * p; q += nt) {
+      const int a = q % p, b = q / p;
+      double acc = 0;
+      for (int k = 0; k < p; ++k) {
+        const long g = gi(k, b);
+        const int qq = k + p * b;
+        acc += Dr[k + p * a] * (rx[g] * sA[qq] + ry[g] * sB[qq]);
+      }
+      double acc2 = 0;
+      for (int k = 0; k < p; ++k) {
+        const long g = gi(a, k);
+        const int qq = a + p * k;
+        acc2 += (sx[g] * sA[qq] + sy[g] * sB[qq]) * Ds[k + p * b];
+      }
+      y[gi(a, b)] = acc + acc2;
+    }
+  }
+}
+
+inline int nblk(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int launch_coords(hipStream_t st, const sem_mesh* m, const double* xe, const double* ye,
+                  int deform, double amp) {
+  hipLaunchKernelGGL(k_coords, dim3(nblk(m->nl)), dim3(256), 0, st, m->f[SEM_X], m->f[SEM_Y], xe,
+                     ye, m->L, (long)m->nl, deform, amp);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_geometry(hipStream_t st, sem_mesh* m) {
+  double** f = m->f;
+  hipLaunchKernelGGL(k_geometry, dim3(nblk(m->nl)), dim3(256), 0, st, f[SEM_X], f[SEM_Y], m->Dr,
+                     m->Ds, m->wr, m->ws, m->nr, m->ns, m->L, (long)m->nl, f[SEM_JAC],
+                     f[SEM_JACI], f[SEM_RX], f[SEM_RY], f[SEM_SX], f[SEM_SY], f[SEM_B],
+                     f[SEM_BI], f[SEM_G11], f[SEM_G12], f[SEM_G22]);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_mult(hipStream_t st, const sem_mesh* m) {
+  hipLaunchKernelGGL(k_mult, dim3(nblk(m->nl)), dim3(256), 0, st, m->f[SEM_MULT], m->nr, m->ns,
+                     m->L, m->NC, m->per_x, m->per_y, (long)m->nl);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_mask_fill(hipStream_t st, const Layout& g, int bits, double* M) {
+  const long n = (long)g.L * g.NC;
+  hipLaunchKernelGGL(k_mask_fill, dim3(nblk(n)), dim3(256), 0, st, M, g.L, g.NC, bits, n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_mask_bc(hipStream_t st, const Layout& g, int bits, const double* u, double* y) {
+  const long n = (long)g.L * g.NC;
+  hipLaunchKernelGGL(k_mask_bc, dim3(nblk(n)), dim3(256), 0, st, u, y, g.L, g.NC, bits, n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_mask(hipStream_t st, int64_t n, const double* M, const double* u, double* y) {
+  hipLaunchKernelGGL(k_mul, dim3(nblk(n)), dim3(256), 0, st, M, u, y, (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_mul(hipStream_t st, int64_t n, const double* a, const double* u, double* y) {
+  return launch_mask(st, n, a, u, y);
+}
+
+int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* y) {
+  hipLaunchKernelGGL((k_gsmap<double>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
+                     map->seg, u, y, (long)map->n_global);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y) {
+  hipLaunchKernelGGL((k_gsmap<float>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
+                     map->seg, u, y, (long)map->n_global);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_convert_f64_f32(hipStream_t st, int64_t n, const double* x, float* y) {
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n)), dim3(256), 0, st, x, y, (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+int launch_convert_f32_f64(hipStream_t st, int64_t n, const float* x, double* y) {
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(nblk(n)), dim3(256), 0, st, x, y, (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_abu(hipStream_t st, int m, int n, const double* u, const double* As, int ma, int na,
+               const double* Br, int mb, int nb, int trans, double* tmp, double* y) {
+  // with trans the operands are read as their transposes: Br' is (nb x mb)
+  const int rmb = trans ? nb : mb, rnb = trans ? mb : nb;
+  const int rma = trans ? na : ma, rna = trans ? ma : na;
+  const double* cur = u;
+  int m1 = m;
+  if (Br) {
+    m1 = (m / rnb) * rmb;
+    double* dst = As ? tmp : y;
+    hipLaunchKernelGGL(k_abu_r, dim3(nblk((long)m1 * n)), dim3(256), 0, st, u, Br, m, n, rmb, rnb,
+                       trans, dst);
+    cur = dst;
+  }
+  if (As) {
+    const int n2 = (n / rna) * rma;
+    hipLaunchKernelGGL(k_abu_s, dim3(nblk((long)m1 * n2)), dim3(256), 0, st, cur, As, m1, n, rma,
+                       rna, trans, y);
+  }
+  if (!Br && !As) SEM_HIP(hipMemcpyAsync(y, u, sizeof(double) * (size_t)m * n,
+                                         hipMemcpyDeviceToDevice, st));
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* b, double* part,
+                int nparts, double* out) {
+  const long n = (long)g.L * g.NC;
+  hipLaunchKernelGGL(k_wdot_part, dim3(nparts), dim3(256), 0, st, a, b, g.P, g.L, g.NC, g.per_x,
+                     g.per_y, n, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_norm_inf(hipStream_t st, int64_t n, const double* a, double* part, int nparts,
+                    double* out) {
+  hipLaunchKernelGGL(k_absmax_part, dim3(nparts), dim3(256), 0, st, a, (long)n, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 1, out);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_grad(hipStream_t st, const sem_mesh* m, const double* u, double* ux, double* uy) {
+  double* const* f = m->f;
+  hipLaunchKernelGGL(k_grad, dim3(nblk(m->nl)), dim3(256), 0, st, u, m->Dr, m->Ds, f[SEM_RX],
+                     f[SEM_RY], f[SEM_SX], f[SEM_SY], m->nr, m->ns, m->L, (long)m->nl, ux, uy);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_grad_t(hipStream_t st, const sem_mesh* m, const double* vx, const double* vy,
+                  double* y) {
+  double* const* f = m->f;
+  hipLaunchKernelGGL(k_grad_t, dim3(nblk(m->nl)), dim3(256), 0, st, vx, vy, m->Dr, m->Ds,
+                     f[SEM_RX], f[SEM_RY], f[SEM_SX], f[SEM_SY], m->nr, m->ns, m->L, (long)m->nl,
+                     y);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_adjoint_dnu(hipStream_t st, const sem_mesh* m, const double* u, const double* lam,
+                       double* g) {
+  double* const* f = m->f;
+  hipLaunchKernelGGL(k_adjoint_dnu, dim3(nblk(m->nl)), dim3(256), 0, st, u, lam, m->Dr, m->Ds,
+                     f[SEM_RX], f[SEM_RY], f[SEM_SX], f[SEM_SY], f[SEM_B], m->nr, m->ns, m->L,
+                     (long)m->nl, g);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_advect(hipStream_t st, const sem_mesh* mV, const sem_mesh* mD, const double* J,
+                  const double* T, const double* vx, const double* vy, double* y,
+                  int transpose) {
+  const int p = mV->nr, pd = mD->nr;
+  const int E = mV->Ex * mV->Ey;
+  double* const* f = mV->f;
+  // J holds Jr (pd x p) followed by Js (pd x p), column-major
+  hipLaunchKernelGGL(k_advect, dim3(E), dim3(256), 0, st, T, vx, vy, mV->Dr, mV->Ds, f[SEM_RX],
+                     f[SEM_RY], f[SEM_SX], f[SEM_SY], mD->f[SEM_B], J, J + pd * p, p, pd, mV->Ex,
+                     mV->Ey, y, transpose);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+}  // namespace sem

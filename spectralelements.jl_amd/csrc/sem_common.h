@@ -1,0 +1,202 @@
+// Internal declarations shared by the host API and the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/sem_abi.h"
+
+namespace sem {
+
+void set_error(const char* fmt, ...);
+
+#define SEM_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::sem::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                  \
+      return SEM_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#define SEM_TRY(expr)            \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ < 0) return s_;       \
+  } while (0)
+
+#define SEM_ARG(cond, ...)              \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::sem::set_error(__VA_ARGS__);    \
+      return SEM_ERR_ARG;               \
+    }                                   \
+  } while (0)
+
+constexpr int kMaxP = 16;
+
+// Geometry of the structured local layout shared by every kernel: column-major
+// (L = nr*Ex rows, NC = ns*Ey columns); element (ex, ey) node (a, b) sits at
+// (ex*nr + a) + L*(ey*ns + b). The fused kernels require nr == ns == P.
+struct Layout {
+  int P;        // points per direction (nr == ns)
+  int Ex, Ey;
+  int L, NC;    // rows, columns
+  int per_x, per_y;
+};
+
+}  // namespace sem
+
+struct sem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;          // device workspace, grown on demand
+  size_t ws_bytes = 0;
+  void* pinned = nullptr;      // small pinned host buffer for scalar readback
+  size_t pinned_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct sem_gsmap {
+  int64_t n_local = 0, n_global = 0;
+  int32_t* perm = nullptr;     // local indices sorted by global id
+  int32_t* seg = nullptr;      // n_global + 1 offsets into perm
+  int device = 0;
+};
+
+struct sem_mesh {
+  int nr = 0, ns = 0, Ex = 0, Ey = 0, per_x = 0, per_y = 0;
+  int L = 0, NC = 0;
+  int64_t nl = 0, ng = 0;
+  int device = 0;
+  double* f[SEM_NUM_MESH_FIELDS] = {};
+  double *Dr = nullptr, *Ds = nullptr, *zr = nullptr, *wr = nullptr, *zs = nullptr, *ws = nullptr;
+  float *Dr32 = nullptr, *Ds32 = nullptr;
+  std::vector<double> hDr, hDs, hzr, hwr, hzs, hws;
+  int32_t* l2g = nullptr;
+  sem_gsmap* map = nullptr;
+  sem::Layout layout() const { return sem::Layout{nr, Ex, Ey, L, NC, per_x, per_y}; }
+};
+
+struct sem_operator {
+  const sem_mesh* mesh = nullptr;
+  int precision = SEM_F64;
+  sem_op_desc desc{};
+  // factors in operator precision (device); owned[] marks library allocations
+  void* G11 = nullptr;
+  void* G12 = nullptr;
+  void* G22 = nullptr;
+  void* B = nullptr;
+  void* nu_f = nullptr;
+  void* k_f = nullptr;
+  bool owned[6] = {false, false, false, false, false, false};
+  bool has_mass = false;
+  // MFMA operand tables (FP16/BF16 bit patterns) for the contraction matrices
+  void* Dmfma = nullptr;
+};
+
+namespace sem {
+
+int ensure_ws(sem_ctx* ctx, size_t bytes);
+int ensure_pinned(sem_ctx* ctx, size_t bytes);
+
+// ---- kernel launchers (defined in the .hip translation units)
+int launch_coords(hipStream_t st, const sem_mesh* m, const double* xe, const double* ye,
+                  int deform, double amp);
+int launch_geometry(hipStream_t st, sem_mesh* m);
+int launch_mult(hipStream_t st, const sem_mesh* m);
+int launch_mask_fill(hipStream_t st, const Layout& g, int bits, double* M);
+
+struct ApplyArgs {
+  const void* u;
+  void* y;
+  const void* G11;
+  const void* G12;
+  const void* G22;
+  const void* B;
+  const void* Dr;
+  const void* Ds;
+  const void* nu_f;
+  const void* k_f;
+  double nu, k;
+  int has_mass;
+  int bc;
+  int assemble;
+};
+template <typename T>
+int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);
+
+// Device-resident PCG scalars (pcg.jl:23-55). t[k & 1] holds t = <r, h> of
+// iteration k; done/iters/converged/res are set once, by the first kernel of
+// the iteration that finds norm(r, Inf) <= tol or k - 1 == maxiter.
+struct PcgState {
+  int done;
+  int iters;
+  int converged;
+  int pad;
+  double res;
+  double t[2];
+};
+
+struct PcgDev {
+  const double* part_prev;  // 2*n3 partials: <r, h> sums then norm(r, Inf) maxima
+  int n3;
+  double* Q;                // partials of <u, A u> from the apply kernels
+  PcgState* state;
+  double tol;
+  int maxiter;
+  int k;                    // 1-based iteration being performed
+};
+
+// Launch geometry of the fused apply: nk1 strip workgroups, nk2 seam blocks.
+struct AxPlan {
+  int CX, nstrips, S, nsegs, nk1, nk2;
+  long nxitems, total;
+};
+AxPlan make_plan(const Layout& g, bool assemble);
+
+struct PcgLaunch {
+  const void* r;  // recursive residual
+  const void* d;  // diagonal preconditioner (NULL = identity)
+  void* p;        // search direction, updated in place to z + beta p
+  PcgDev dev;
+};
+template <typename T>
+int launch_apply_pcg(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch& pc);
+
+int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* y);
+int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y);
+int launch_mask(hipStream_t st, int64_t n, const double* M, const double* u, double* y);
+int launch_mask_bc(hipStream_t st, const Layout& g, int bits, const double* u, double* y);
+int launch_mul(hipStream_t st, int64_t n, const double* a, const double* u, double* y);
+int launch_abu(hipStream_t st, int m, int n, const double* u, const double* As, int ma,
+               int na, const double* Br, int mb, int nb, int trans, double* tmp, double* y);
+int launch_convert_f64_f32(hipStream_t st, int64_t n, const double* x, float* y);
+int launch_convert_f32_f64(hipStream_t st, int64_t n, const float* x, double* y);
+int launch_jacobi_local(hipStream_t st, const sem_operator* op, double* dl);
+int launch_inv_masked(hipStream_t st, const Layout& g, int bits, const double* d, double* dinv);
+
+// reductions: results left in device memory at out[0]
+int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* b, double* part,
+                int nparts, double* out);
+int launch_norm_inf(hipStream_t st, int64_t n, const double* a, double* part, int nparts,
+                    double* out);
+
+int launch_grad(hipStream_t st, const sem_mesh* m, const double* u, double* ux, double* uy);
+int launch_grad_t(hipStream_t st, const sem_mesh* m, const double* vx, const double* vy,
+                  double* y);
+int launch_adjoint_dnu(hipStream_t st, const sem_mesh* m, const double* u, const double* lam,
+                       double* g);
+int launch_advect(hipStream_t st, const sem_mesh* mV, const sem_mesh* mD, const double* J,
+                  const double* T, const double* vx, const double* vy, double* y, int transpose);
+
+// PCG driver (k_pcg.hip)
+int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,
+            void* x, double tol, int maxiter, sem_pcg_info* info);
+
+}  // namespace sem

@@ -1,0 +1,681 @@
+// Host side of libsemhip.so: the C ABI declared in include/sem_abi.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "sem_common.h"
+#include "sem_l0.h"
+#include "_gen/gll_dtables.h"
+
+namespace sem {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int ensure_ws(sem_ctx* ctx, size_t bytes) {
+  if (ctx->ws_bytes >= bytes) return SEM_OK;
+  if (ctx->ws) {
+    SEM_HIP(hipStreamSynchronize(ctx->stream));
+    SEM_HIP(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+  }
+  SEM_HIP(hipMalloc(&ctx->ws, bytes));
+  ctx->ws_bytes = bytes;
+  return SEM_OK;
+}
+
+int ensure_pinned(sem_ctx* ctx, size_t bytes) {
+  if (ctx->pinned_bytes >= bytes) return SEM_OK;
+  if (ctx->pinned) {
+    SEM_HIP(hipStreamSynchronize(ctx->stream));
+    SEM_HIP(hipHostFree(ctx->pinned));
+    ctx->pinned = nullptr;
+  }
+  SEM_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+  ctx->pinned_bytes = bytes;
+  return SEM_OK;
+}
+
+namespace {
+
+template <typename T>
+int upload(hipStream_t st, T** dst, const T* src, size_t n) {
+  SEM_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(n, 1)));
+  SEM_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+  SEM_HIP(hipStreamSynchronize(st));
+  return SEM_OK;
+}
+
+int build_gsmap(sem_ctx* ctx, int64_t nl, const int32_t* l2g, int64_t ng, sem_gsmap** out) {
+  std::vector<int32_t> seg(ng + 1, 0), perm(nl), fill(ng, 0);
+  for (int64_t l = 0; l < nl; ++l) {
+    const int32_t g = l2g[l];
+    if (g < 0 || g >= ng) {
+      set_error("gsmap: global id %d of local %lld outside [0, %lld)", g, (long long)l,
+                (long long)ng);
+      return SEM_ERR_ARG;
+    }
+    seg[g + 1]++;
+  }
+  for (int64_t g = 0; g < ng; ++g) seg[g + 1] += seg[g];
+  for (int64_t l = 0; l < nl; ++l) {  // ascending l: stable within a segment
+    const int32_t g = l2g[l];
+    perm[seg[g] + fill[g]++] = (int32_t)l;
+  }
+  sem_gsmap* m = new (std::nothrow) sem_gsmap();
+  if (!m) {
+    set_error("out of host memory");
+    return SEM_ERR_ARG;
+  }
+  m->n_local = nl;
+  m->n_global = ng;
+  hipGetDevice(&m->device);
+  int s = upload(ctx->stream, &m->perm, perm.data(), (size_t)nl);
+  if (s == SEM_OK) s = upload(ctx->stream, &m->seg, seg.data(), (size_t)ng + 1);
+  if (s != SEM_OK) {
+    delete m;
+    return s;
+  }
+  *out = m;
+  return SEM_OK;
+}
+
+// The fused kernels use compile-time GLL tables; they must equal Mesh.Dr.
+template <int P>
+bool table_equal(const double* D) {
+  for (int i = 0; i < P * P; ++i)
+    if (gll_d<P>(i) != D[i]) return false;
+  return true;
+}
+
+bool gll_table_matches(int P, const double* D) {
+  switch (P) {
+    case 2: return table_equal<2>(D);
+    case 3: return table_equal<3>(D);
+    case 4: return table_equal<4>(D);
+    case 5: return table_equal<5>(D);
+    case 6: return table_equal<6>(D);
+    case 7: return table_equal<7>(D);
+    case 8: return table_equal<8>(D);
+    case 9: return table_equal<9>(D);
+    case 10: return table_equal<10>(D);
+    case 11: return table_equal<11>(D);
+    case 12: return table_equal<12>(D);
+    case 13: return table_equal<13>(D);
+    case 14: return table_equal<14>(D);
+    case 15: return table_equal<15>(D);
+    case 16: return table_equal<16>(D);
+    default: return false;
+  }
+}
+
+int check_square(const sem_mesh* m) {
+  if (m->nr != m->ns) {
+    set_error("fused operators need nr == ns (got %d, %d)", m->nr, m->ns);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  if (m->nr < 2 || m->nr > kMaxP) {
+    set_error("fused operators support 2 <= nr <= %d (got %d)", kMaxP, m->nr);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  if (!gll_table_matches(m->nr, m->hDr.data())) {
+    set_error("internal: compile-time GLL table differs from Mesh.Dr for nr = %d", m->nr);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  return SEM_OK;
+}
+
+ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
+  const sem_mesh* m = op->mesh;
+  ApplyArgs a{};
+  a.u = u;
+  a.y = y;
+  a.G11 = op->G11;
+  a.G12 = op->G12;
+  a.G22 = op->G22;
+  a.B = op->B;
+  const bool f64 = op->precision == SEM_F64;
+  a.Dr = f64 ? (const void*)m->Dr : (const void*)m->Dr32;
+  a.Ds = f64 ? (const void*)m->Ds : (const void*)m->Ds32;
+  a.nu_f = op->nu_f;
+  a.k_f = op->k_f;
+  a.nu = op->desc.nu;
+  a.k = op->desc.k;
+  a.has_mass = op->has_mass ? 1 : 0;
+  a.bc = op->desc.assemble ? op->desc.bc_bits : 0;
+  a.assemble = op->desc.assemble;
+  return a;
+}
+
+void free_mesh(sem_mesh* m) {
+  if (!m) return;
+  for (int i = 0; i < SEM_NUM_MESH_FIELDS; ++i)
+    if (m->f[i]) hipFree(m->f[i]);
+  void* ptrs[] = {m->Dr, m->Ds, m->zr, m->wr, m->zs, m->ws, m->Dr32, m->Ds32, m->l2g};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (m->map) {
+    hipFree(m->map->perm);
+    hipFree(m->map->seg);
+    delete m->map;
+  }
+  delete m;
+}
+
+}  // namespace
+}  // namespace sem
+
+using namespace sem;
+
+extern "C" {
+
+int sem_abi_version(void) { return SEM_ABI_VERSION; }
+
+const char* sem_last_error(void) { return sem::g_err.c_str(); }
+
+int sem_ctx_create(int device, void* stream, sem_ctx** out) {
+  SEM_ARG(out, "sem_ctx_create: out is NULL");
+  int ndev = 0;
+  SEM_HIP(hipGetDeviceCount(&ndev));
+  SEM_ARG(device >= 0 && device < ndev, "sem_ctx_create: device %d not in [0, %d)", device, ndev);
+  SEM_HIP(hipSetDevice(device));
+  sem_ctx* c = new (std::nothrow) sem_ctx();
+  SEM_ARG(c, "out of host memory");
+  c->device = device;
+  c->stream = static_cast<hipStream_t>(stream);
+  SEM_HIP(hipEventCreate(&c->ev0));
+  SEM_HIP(hipEventCreate(&c->ev1));
+  *out = c;
+  return SEM_OK;
+}
+
+int sem_ctx_destroy(sem_ctx* ctx) {
+  if (!ctx) return SEM_OK;
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->ws) hipFree(ctx->ws);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  delete ctx;
+  return SEM_OK;
+}
+
+int sem_ctx_set_stream(sem_ctx* ctx, void* stream) {
+  SEM_ARG(ctx, "ctx is NULL");
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return SEM_OK;
+}
+
+int sem_ctx_synchronize(sem_ctx* ctx) {
+  SEM_ARG(ctx, "ctx is NULL");
+  SEM_HIP(hipStreamSynchronize(ctx->stream));
+  return SEM_OK;
+}
+
+int sem_gausslobatto(int n, double* z, double* w) {
+  SEM_ARG(n >= 2 && z && w, "sem_gausslobatto: need n >= 2 and output arrays");
+  gll(n, z, w);
+  return SEM_OK;
+}
+
+int sem_derivmat(int n, const double* x, double* D) {
+  SEM_ARG(n >= 1 && x && D, "sem_derivmat: bad arguments");
+  derivmat(n, x, D);
+  return SEM_OK;
+}
+
+int sem_interpmat(int no, const double* xo, int ni, const double* xi, double* J) {
+  SEM_ARG(no >= 1 && ni >= 1 && xo && xi && J, "sem_interpmat: bad arguments");
+  interpmat(no, xo, ni, xi, J);
+  return SEM_OK;
+}
+
+int sem_semmesh(int E, int n, double* z, double* w) {
+  SEM_ARG(E >= 1 && n >= 2 && z, "sem_semmesh: bad arguments");
+  semmesh(E, n, z, w);
+  return SEM_OK;
+}
+
+int sem_mesh_create(sem_ctx* ctx, int nr, int ns, int Ex, int Ey, int periodic_x, int periodic_y,
+                    int deform_kind, double deform_param, sem_mesh** out) {
+  SEM_ARG(ctx && out, "sem_mesh_create: ctx/out is NULL");
+  SEM_ARG(nr >= 2 && ns >= 2 && nr <= 64 && ns <= 64, "sem_mesh_create: need 2 <= nr, ns <= 64");
+  SEM_ARG(Ex >= 1 && Ey >= 1, "sem_mesh_create: need Ex, Ey >= 1");
+  SEM_ARG(deform_kind >= SEM_DEFORM_NONE && deform_kind <= SEM_DEFORM_AFFINE01,
+          "sem_mesh_create: unknown deform kind %d", deform_kind);
+  const int64_t L = (int64_t)nr * Ex, NC = (int64_t)ns * Ey;
+  SEM_ARG(L * NC < (int64_t(1) << 31), "sem_mesh_create: %lld local nodes exceed int32 indexing",
+          (long long)(L * NC));
+  SEM_HIP(hipSetDevice(ctx->device));
+  sem_mesh* m = new (std::nothrow) sem_mesh();
+  SEM_ARG(m, "out of host memory");
+  m->nr = nr;
+  m->ns = ns;
+  m->Ex = Ex;
+  m->Ey = Ey;
+  m->per_x = periodic_x ? 1 : 0;
+  m->per_y = periodic_y ? 1 : 0;
+  m->L = (int)L;
+  m->NC = (int)NC;
+  m->nl = L * NC;
+  m->device = ctx->device;
+  const int64_t nxg = (int64_t)Ex * (nr - 1) + (m->per_x ? 0 : 1);
+  const int64_t nyg = (int64_t)Ey * (ns - 1) + (m->per_y ? 0 : 1);
+  m->ng = nxg * nyg;
+  hipStream_t st = ctx->stream;
+  int s = SEM_OK;
+  auto fail = [&](int code) {
+    free_mesh(m);
+    return code;
+  };
+  m->hzr.resize(nr);
+  m->hwr.resize(nr);
+  m->hzs.resize(ns);
+  m->hws.resize(ns);
+  gll(nr, m->hzr.data(), m->hwr.data());
+  gll(ns, m->hzs.data(), m->hws.data());
+  m->hDr.resize((size_t)nr * nr);
+  m->hDs.resize((size_t)ns * ns);
+  derivmat(nr, m->hzr.data(), m->hDr.data());
+  derivmat(ns, m->hzs.data(), m->hDs.data());
+  for (int i = 0; i < SEM_NUM_MESH_FIELDS; ++i) {
+    if (hipMalloc(&m->f[i], sizeof(double) * m->nl) != hipSuccess) {
+      set_error("sem_mesh_create: hipMalloc of %lld doubles failed", (long long)m->nl);
+      return fail(SEM_ERR_HIP);
+    }
+  }
+  if ((s = upload(st, &m->Dr, m->hDr.data(), m->hDr.size())) != SEM_OK) return fail(s);
+  if ((s = upload(st, &m->Ds, m->hDs.data(), m->hDs.size())) != SEM_OK) return fail(s);
+  if ((s = upload(st, &m->zr, m->hzr.data(), nr)) != SEM_OK) return fail(s);
+  if ((s = upload(st, &m->wr, m->hwr.data(), nr)) != SEM_OK) return fail(s);
+  if ((s = upload(st, &m->zs, m->hzs.data(), ns)) != SEM_OK) return fail(s);
+  if ((s = upload(st, &m->ws, m->hws.data(), ns)) != SEM_OK) return fail(s);
+  {
+    std::vector<float> a(m->hDr.begin(), m->hDr.end()), b(m->hDs.begin(), m->hDs.end());
+    if ((s = upload(st, &m->Dr32, a.data(), a.size())) != SEM_OK) return fail(s);
+    if ((s = upload(st, &m->Ds32, b.data(), b.size())) != SEM_OK) return fail(s);
+  }
+  // coordinates (semmesh + ndgrid + deform) and FP64 geometry on the GPU
+  std::vector<double> xe(L), ye(NC);
+  semmesh(Ex, nr, xe.data(), nullptr);
+  semmesh(Ey, ns, ye.data(), nullptr);
+  double *dxe = nullptr, *dye = nullptr;
+  if ((s = upload(st, &dxe, xe.data(), xe.size())) != SEM_OK) return fail(s);
+  if ((s = upload(st, &dye, ye.data(), ye.size())) != SEM_OK) {
+    hipFree(dxe);
+    return fail(s);
+  }
+  s = launch_coords(st, m, dxe, dye, deform_kind, deform_param);
+  if (s == SEM_OK) s = launch_geometry(st, m);
+  if (s == SEM_OK) s = launch_mult(st, m);
+  hipStreamSynchronize(st);
+  hipFree(dxe);
+  hipFree(dye);
+  if (s != SEM_OK) return fail(s);
+  // local-to-global numbering (semq.jl:6-25 in both directions, mesh.jl:87-89)
+  std::vector<int32_t> l2g(m->nl);
+  for (int64_t j = 0; j < NC; ++j) {
+    int64_t gy = (j / ns) * (ns - 1) + (j % ns);
+    if (m->per_y) gy %= nyg;
+    for (int64_t i = 0; i < L; ++i) {
+      int64_t gx = (i / nr) * (nr - 1) + (i % nr);
+      if (m->per_x) gx %= nxg;
+      l2g[i + L * j] = (int32_t)(gx + nxg * gy);
+    }
+  }
+  if ((s = upload(st, &m->l2g, l2g.data(), l2g.size())) != SEM_OK) return fail(s);
+  if ((s = build_gsmap(ctx, m->nl, l2g.data(), m->ng, &m->map)) != SEM_OK) return fail(s);
+  *out = m;
+  return SEM_OK;
+}
+
+int sem_mesh_set_coords(sem_ctx* ctx, sem_mesh* m, const double* x, const double* y) {
+  SEM_ARG(ctx && m && x && y, "sem_mesh_set_coords: NULL argument");
+  hipStream_t st = ctx->stream;
+  if (x != m->f[SEM_X])
+    SEM_HIP(hipMemcpyAsync(m->f[SEM_X], x, sizeof(double) * m->nl, hipMemcpyDeviceToDevice, st));
+  if (y != m->f[SEM_Y])
+    SEM_HIP(hipMemcpyAsync(m->f[SEM_Y], y, sizeof(double) * m->nl, hipMemcpyDeviceToDevice, st));
+  SEM_TRY(launch_geometry(st, m));
+  return SEM_OK;
+}
+
+int sem_mesh_destroy(sem_mesh* m) {
+  if (m) hipDeviceSynchronize();
+  free_mesh(m);
+  return SEM_OK;
+}
+
+int sem_mesh_sizes(const sem_mesh* m, int* nr, int* ns, int* Ex, int* Ey, int64_t* nl,
+                   int64_t* ng) {
+  SEM_ARG(m, "mesh is NULL");
+  if (nr) *nr = m->nr;
+  if (ns) *ns = m->ns;
+  if (Ex) *Ex = m->Ex;
+  if (Ey) *Ey = m->Ey;
+  if (nl) *nl = m->nl;
+  if (ng) *ng = m->ng;
+  return SEM_OK;
+}
+
+int sem_mesh_field(const sem_mesh* m, int which, const double** p) {
+  SEM_ARG(m && p, "NULL argument");
+  SEM_ARG(which >= 0 && which < SEM_NUM_MESH_FIELDS, "sem_mesh_field: bad field id %d", which);
+  *p = m->f[which];
+  return SEM_OK;
+}
+
+int sem_mesh_dmat(const sem_mesh* m, const double** Dr, const double** Ds) {
+  SEM_ARG(m, "mesh is NULL");
+  if (Dr) *Dr = m->Dr;
+  if (Ds) *Ds = m->Ds;
+  return SEM_OK;
+}
+
+int sem_mesh_gll(const sem_mesh* m, const double** zr, const double** wr, const double** zs,
+                 const double** ws) {
+  SEM_ARG(m, "mesh is NULL");
+  if (zr) *zr = m->zr;
+  if (wr) *wr = m->wr;
+  if (zs) *zs = m->zs;
+  if (ws) *ws = m->ws;
+  return SEM_OK;
+}
+
+int sem_mesh_l2g(const sem_mesh* m, const int32_t** p) {
+  SEM_ARG(m && p, "NULL argument");
+  *p = m->l2g;
+  return SEM_OK;
+}
+
+int sem_mask_bits(const sem_mesh* m, const char* bc, int* bits) {
+  SEM_ARG(m && bc && bits, "NULL argument");
+  int b = 0;
+  for (int q = 0; q < 4; ++q) {
+    SEM_ARG(bc[q] == 'D' || bc[q] == 'N', "sem_mask_bits: bc[%d] = '%c' is not 'D' or 'N'", q,
+            bc[q]);
+    if (bc[q] == 'D') b |= (1 << q);
+  }
+  if (m->per_x) b &= ~(SEM_BC_XMIN | SEM_BC_XMAX);
+  if (m->per_y) b &= ~(SEM_BC_YMIN | SEM_BC_YMAX);
+  *bits = b;
+  return SEM_OK;
+}
+
+int sem_mask_fill(sem_ctx* ctx, const sem_mesh* m, int bits, double* M) {
+  SEM_ARG(ctx && m && M, "NULL argument");
+  return launch_mask_fill(ctx->stream, m->layout(), bits, M);
+}
+
+int sem_gsmap_create(sem_ctx* ctx, int64_t nl, const int32_t* l2g, int64_t ng, sem_gsmap** out) {
+  SEM_ARG(ctx && l2g && out && nl >= 1 && ng >= 1, "sem_gsmap_create: bad arguments");
+  SEM_ARG(nl < (int64_t(1) << 31), "sem_gsmap_create: too many local nodes");
+  return build_gsmap(ctx, nl, l2g, ng, out);
+}
+
+int sem_gsmap_destroy(sem_gsmap* map) {
+  if (!map) return SEM_OK;
+  hipDeviceSynchronize();
+  hipFree(map->perm);
+  hipFree(map->seg);
+  delete map;
+  return SEM_OK;
+}
+
+int sem_gsmap_apply(sem_ctx* ctx, const sem_gsmap* map, const double* u, double* y) {
+  SEM_ARG(ctx && map && u && y, "NULL argument");
+  return launch_gsmap(ctx->stream, map, u, y);
+}
+
+int sem_gather_scatter(sem_ctx* ctx, const sem_mesh* m, const double* u, double* y) {
+  SEM_ARG(ctx && m && u && y, "NULL argument");
+  return launch_gsmap(ctx->stream, m->map, u, y);
+}
+
+int sem_gather_scatter_f32(sem_ctx* ctx, const sem_mesh* m, const float* u, float* y) {
+  SEM_ARG(ctx && m && u && y, "NULL argument");
+  return launch_gsmap_f32(ctx->stream, m->map, u, y);
+}
+
+int sem_mask(sem_ctx* ctx, int64_t n, const double* M, const double* u, double* y) {
+  SEM_ARG(ctx && u && y && n >= 0, "bad arguments");
+  return launch_mask(ctx->stream, n, M, u, y);
+}
+
+int sem_mask_bc(sem_ctx* ctx, const sem_mesh* m, int bits, const double* u, double* y) {
+  SEM_ARG(ctx && m && u && y, "NULL argument");
+  return launch_mask_bc(ctx->stream, m->layout(), bits, u, y);
+}
+
+int sem_mass(sem_ctx* ctx, const sem_mesh* m, const double* u, double* y) {
+  SEM_ARG(ctx && m && u && y, "NULL argument");
+  return launch_mul(ctx->stream, m->nl, m->f[SEM_B], u, y);
+}
+
+int sem_abu(sem_ctx* ctx, int m, int n, const double* u, const double* As, int ma, int na,
+            const double* Br, int mb, int nb, int trans, double* y) {
+  SEM_ARG(ctx && u && y && m >= 1 && n >= 1, "sem_abu: bad arguments");
+  const int rnb = trans ? mb : nb, rmb = trans ? nb : mb;
+  const int rna = trans ? ma : na;
+  if (Br) SEM_ARG(rnb >= 1 && m % rnb == 0, "sem_abu: size(Br,2)=%d does not divide m=%d", rnb, m);
+  if (As) SEM_ARG(rna >= 1 && n % rna == 0, "sem_abu: size(As,2)=%d does not divide n=%d", rna, n);
+  size_t tmpn = 0;
+  if (Br && As) tmpn = (size_t)(m / rnb) * rmb * n;
+  double* tmp = nullptr;
+  if (tmpn) {
+    SEM_TRY(ensure_ws(ctx, sizeof(double) * tmpn));
+    tmp = static_cast<double*>(ctx->ws);
+  }
+  return launch_abu(ctx->stream, m, n, u, As, ma, na, Br, mb, nb, trans, tmp, y);
+}
+
+int sem_lapl(sem_ctx* ctx, const sem_mesh* m, const double* u, double* y) {
+  return sem_hlmz(ctx, m, u, 1.0, nullptr, 0.0, nullptr, y);
+}
+
+int sem_hlmz(sem_ctx* ctx, const sem_mesh* m, const double* u, double nu, const double* nu_f,
+             double k, const double* k_f, double* y) {
+  SEM_ARG(ctx && m && u && y, "NULL argument");
+  SEM_TRY(check_square(m));
+  ApplyArgs a{};
+  a.u = u;
+  a.y = y;
+  a.G11 = m->f[SEM_G11];
+  a.G12 = m->f[SEM_G12];
+  a.G22 = m->f[SEM_G22];
+  a.B = m->f[SEM_B];
+  a.Dr = m->Dr;
+  a.Ds = m->Ds;
+  a.nu_f = nu_f;
+  a.k_f = k_f;
+  a.nu = nu;
+  a.k = k;
+  a.has_mass = (k != 0.0 || k_f) ? 1 : 0;
+  a.bc = 0;
+  a.assemble = 0;
+  return launch_apply<double>(ctx->stream, m->layout(), a);
+}
+
+int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc, int precision,
+                        sem_operator** out) {
+  SEM_ARG(ctx && m && desc && out, "NULL argument");
+  SEM_TRY(check_square(m));
+  SEM_ARG(precision >= SEM_F64 && precision <= SEM_BF16_MFMA, "bad precision %d", precision);
+  if (precision == SEM_F16_MFMA || precision == SEM_BF16_MFMA) {
+    if (m->nr != 16) {
+      set_error("MFMA operators need nr == ns == 16 (got %d)", m->nr);
+      return SEM_ERR_UNSUPPORTED;
+    }
+  }
+  sem_operator* op = new (std::nothrow) sem_operator();
+  SEM_ARG(op, "out of host memory");
+  op->mesh = m;
+  op->precision = precision;
+  op->desc = *desc;
+  op->has_mass = desc->k != 0.0 || desc->k_f;
+  const double* src[6] = {desc->G11 ? desc->G11 : m->f[SEM_G11],
+                          desc->G12 ? desc->G12 : m->f[SEM_G12],
+                          desc->G22 ? desc->G22 : m->f[SEM_G22],
+                          desc->B ? desc->B : m->f[SEM_B], desc->nu_f, desc->k_f};
+  void** dst[6] = {&op->G11, &op->G12, &op->G22, &op->B, &op->nu_f, &op->k_f};
+  for (int q = 0; q < 6; ++q) {
+    if (!src[q]) continue;
+    if (precision == SEM_F64) {
+      *dst[q] = const_cast<double*>(src[q]);
+    } else {
+      float* f = nullptr;
+      if (hipMalloc(&f, sizeof(float) * m->nl) != hipSuccess) {
+        set_error("sem_operator_create: hipMalloc failed");
+        sem_operator_destroy(op);
+        return SEM_ERR_HIP;
+      }
+      op->owned[q] = true;
+      *dst[q] = f;
+      int s = launch_convert_f64_f32(ctx->stream, m->nl, src[q], f);
+      if (s != SEM_OK) {
+        sem_operator_destroy(op);
+        return s;
+      }
+    }
+  }
+  *out = op;
+  return SEM_OK;
+}
+
+int sem_operator_destroy(sem_operator* op) {
+  if (!op) return SEM_OK;
+  void* ps[6] = {op->G11, op->G12, op->G22, op->B, op->nu_f, op->k_f};
+  bool any = false;
+  for (int q = 0; q < 6; ++q) any = any || op->owned[q];
+  if (any || op->Dmfma) hipDeviceSynchronize();
+  for (int q = 0; q < 6; ++q)
+    if (op->owned[q] && ps[q]) hipFree(ps[q]);
+  if (op->Dmfma) hipFree(op->Dmfma);
+  delete op;
+  return SEM_OK;
+}
+
+int sem_operator_precision(const sem_operator* op, int* precision) {
+  SEM_ARG(op && precision, "NULL argument");
+  *precision = op->precision;
+  return SEM_OK;
+}
+
+int sem_operator_apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y) {
+  SEM_ARG(ctx && op && u && y, "NULL argument");
+  SEM_ARG(u != y, "sem_operator_apply: u and y must not alias");
+  const ApplyArgs a = args_of(op, u, y);
+  switch (op->precision) {
+    case SEM_F64: return launch_apply<double>(ctx->stream, op->mesh->layout(), a);
+    case SEM_F32: return launch_apply<float>(ctx->stream, op->mesh->layout(), a);
+    default:
+      set_error("sem_operator_apply: precision %d not available", op->precision);
+      return SEM_ERR_UNSUPPORTED;
+  }
+}
+
+int sem_operator_jacobi(sem_ctx* ctx, const sem_operator* op, double* dinv) {
+  SEM_ARG(ctx && op && dinv, "NULL argument");
+  const sem_mesh* m = op->mesh;
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * m->nl));
+  double* dl = static_cast<double*>(ctx->ws);
+  SEM_TRY(launch_jacobi_local(ctx->stream, op, dl));
+  SEM_TRY(launch_gsmap(ctx->stream, m->map, dl, dl));
+  SEM_TRY(launch_inv_masked(ctx->stream, m->layout(), op->desc.bc_bits, dl, dinv));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));  // dl lives in the shared workspace
+  return SEM_OK;
+}
+
+int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,
+            void* x, double tol, int maxiter, sem_pcg_info* info) {
+  SEM_ARG(ctx && op && b && x, "NULL argument");
+  SEM_ARG(op->desc.assemble, "sem_pcg needs an assembled operator (desc.assemble = 1)");
+  SEM_ARG(precond == SEM_PRECOND_IDENTITY || (precond == SEM_PRECOND_DIAG && d),
+          "sem_pcg: SEM_PRECOND_DIAG needs the diagonal d");
+  SEM_ARG(maxiter >= 0, "sem_pcg: maxiter < 0");
+  SEM_ARG(tol >= 0.0, "sem_pcg: tol < 0");
+  return pcg_run(ctx, op, precond, d, b, x, tol, maxiter, info);
+}
+
+int sem_wdot(sem_ctx* ctx, const sem_mesh* m, const double* a, const double* b, double* result) {
+  SEM_ARG(ctx && m && a && b && result, "NULL argument");
+  const int np = 256;
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * (np + 1)));
+  double* part = static_cast<double*>(ctx->ws);
+  SEM_TRY(launch_wdot(ctx->stream, m->layout(), a, b, part, np, part + np));
+  SEM_HIP(hipMemcpyAsync(result, part + np, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));
+  return SEM_OK;
+}
+
+int sem_norm_inf(sem_ctx* ctx, int64_t n, const double* a, double* result) {
+  SEM_ARG(ctx && a && result && n >= 0, "bad arguments");
+  const int np = 256;
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * (np + 1)));
+  double* part = static_cast<double*>(ctx->ws);
+  SEM_TRY(launch_norm_inf(ctx->stream, n, a, part, np, part + np));
+  SEM_HIP(hipMemcpyAsync(result, part + np, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));
+  return SEM_OK;
+}
+
+int sem_grad(sem_ctx* ctx, const sem_mesh* m, const double* u, double* ux, double* uy) {
+  SEM_ARG(ctx && m && u && ux && uy, "NULL argument");
+  return launch_grad(ctx->stream, m, u, ux, uy);
+}
+
+int sem_grad_t(sem_ctx* ctx, const sem_mesh* m, const double* vx, const double* vy, double* y) {
+  SEM_ARG(ctx && m && vx && vy && y, "NULL argument");
+  return launch_grad_t(ctx->stream, m, vx, vy, y);
+}
+
+int sem_adjoint_dnu(sem_ctx* ctx, const sem_mesh* m, const double* u, const double* lam,
+                    double* g) {
+  SEM_ARG(ctx && m && u && lam && g, "NULL argument");
+  return launch_adjoint_dnu(ctx->stream, m, u, lam, g);
+}
+
+static int advect_common(sem_ctx* ctx, const sem_mesh* mV, const sem_mesh* mD, const double* T,
+                         const double* vx, const double* vy, double* y, int transpose) {
+  SEM_ARG(ctx && mV && mD && T && vx && vy && y, "NULL argument");
+  SEM_ARG(mV->Ex == mD->Ex && mV->Ey == mD->Ey, "advect: meshes differ in element counts");
+  SEM_ARG(mV->nr == mV->ns && mD->nr == mD->ns, "advect: needs nr == ns on both meshes");
+  SEM_ARG(mV->nr <= kMaxP && mD->nr <= 24, "advect: supports p <= 16 and dealiased p <= 24");
+  const int p = mV->nr, pd = mD->nr;
+  std::vector<double> J(2 * (size_t)pd * p);
+  interpmat(pd, mD->hzr.data(), p, mV->hzr.data(), J.data());
+  interpmat(pd, mD->hzs.data(), p, mV->hzs.data(), J.data() + (size_t)pd * p);
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * J.size()));
+  double* dJ = static_cast<double*>(ctx->ws);
+  SEM_HIP(hipMemcpyAsync(dJ, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  SEM_TRY(launch_advect(ctx->stream, mV, mD, dJ, T, vx, vy, y, transpose));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));  // J lives on this call's stack
+  return SEM_OK;
+}
+
+int sem_advect(sem_ctx* ctx, const sem_mesh* mV, const sem_mesh* mD, const double* T,
+               const double* vx, const double* vy, double* y) {
+  return advect_common(ctx, mV, mD, T, vx, vy, y, 0);
+}
+
+int sem_advect_t(sem_ctx* ctx, const sem_mesh* mV, const sem_mesh* mD, const double* w,
+                 const double* vx, const double* vy, double* y) {
+  return advect_common(ctx, mV, mD, w, vx, vy, y, 1);
+}
+
+}  // extern "C"

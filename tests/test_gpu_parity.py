@@ -1,0 +1,333 @@
+"""GPU parity of the HIP path against the oracle (called through the C ABI
+via the semhip mirror). Integer/structural data (numbering, mask, mult) and
+the assembly order are checked bit-exactly; FP64 operators within 1e-12
+relative; converged solves within the tolerance stated per test."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sem_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(f"[case] {msg}", file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def se():
+    import semhip
+    return semhip
+
+
+@pytest.fixture
+def rows_env():
+    old = os.environ.get("SEM_AX_ROWS")
+    yield
+    if old is None:
+        os.environ.pop("SEM_AX_ROWS", None)
+    else:
+        os.environ["SEM_AX_ROWS"] = old
+
+
+def _pair(se, n, E, per=(False, False), deform="sinsin", Ey=None):
+    Ey = Ey or E
+    if deform == "sinsin":
+        dg, do = se.deform_sinsin(0.1), o.deform_sinsin(0.1)
+    elif deform == "affine":
+        dg, do = se.deform_affine01, o.deform_affine01
+    else:
+        dg, do = se.fixU, o.fixU
+    m = se.Mesh(n, n, E, Ey, per, dg)
+    mo = o.Mesh(n, n, E, Ey, per, do, dense_gs=False)
+    return m, mo
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("n,E,Ey,per,deform", [
+    (9, 4, 4, (False, False), "none"),
+    (8, 3, 2, (True, False), "sinsin"),
+    (5, 2, 3, (True, True), "affine"),
+    (16, 2, 2, (False, True), "sinsin"),
+])
+def test_mesh_setup_matches_oracle(se, n, E, Ey, per, deform):
+    _log(f"mesh n={n} E={E}x{Ey} per={per} {deform}")
+    m, mo = _pair(se, n, E, per, deform, Ey)
+    # fields that vanish identically (ry, sx, G12 on affine maps) are rounding
+    # noise on both sides, so each is scaled by the largest field of its group
+    groups = [["x", "y"], ["Jac"], ["rx", "ry", "sx", "sy"], ["B"], ["G11", "G12", "G22"]]
+    for grp in groups:
+        scale = max(np.abs(getattr(mo, nm)).max() for nm in grp)
+        for name in grp:
+            got = se.to_host(m.field(name))
+            ref = getattr(mo, name)
+            # 1e-12: sums of D entries up to N(N+1)/4 = 60 at N = 15 (the
+            # reference's own GEMM summation order is not pinned)
+            assert np.abs(got - ref).max() <= 1e-12 * scale, name
+    # structural data are bit-exact
+    np.testing.assert_array_equal(se.to_host(m.mult), mo.mult)
+    np.testing.assert_array_equal(se.to_host(m.l2g).astype(np.int64), mo.l2g)
+    assert m.nglobal == mo.nglobal
+    for bc in ("DDDD", "DNDN", "NNDD", "NNNN"):
+        np.testing.assert_array_equal(se.to_host(se.generateMask(bc, m)), o.generateMask(bc, mo))
+
+
+@pytest.mark.parametrize("n,E,Ey,per", [
+    (8, 3, 2, (False, False)), (5, 4, 3, (True, False)), (4, 3, 3, (True, True)),
+    (3, 1, 1, (True, True)), (9, 4, 4, (False, True)),
+])
+def test_gather_scatter_map_is_value_exact(se, n, E, Ey, per):
+    _log(f"gs n={n} E={E}x{Ey} per={per}")
+    m = se.Mesh(n, n, E, Ey, per)
+    mo = o.Mesh(n, n, E, Ey, per)   # dense QQtx/QQty as the reference
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(mo.x.shape)
+    ref = o.gatherScatter(u, mo.QQtx, mo.QQty)
+    got = se.to_host(se.gatherScatter(se.to_device(u), m))
+    np.testing.assert_array_equal(got, ref)
+    # reference's (u, QQtx, QQty) form and the (u, l2g) form
+    np.testing.assert_array_equal(se.gatherScatter(u, mo.QQtx, mo.QQty), ref)
+    np.testing.assert_array_equal(se.gatherScatter(u, mo.l2g), ref)
+    # in place
+    ud = se.to_device(u)
+    se.gatherScatter(ud, m, out=ud)
+    np.testing.assert_array_equal(se.to_host(ud), ref)
+
+
+def test_element_operators_match_oracle(se):
+    _log("element ops")
+    m, mo = _pair(se, 8, 3, (False, False), "sinsin", 2)
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(mo.x.shape)
+    nu = 1.0 + rng.random(mo.x.shape)
+    ud = se.to_device(u)
+    assert _rel(se.to_host(se.lapl(ud, m)), o.lapl(u, mo)) < 1e-12
+    # B .* u is exact given B; B itself agrees with the oracle to rounding
+    np.testing.assert_array_equal(se.to_host(se.mass(ud, m)), se.to_host(m.B) * u)
+    assert _rel(se.to_host(se.mass(ud, m)), o.mass(u, mo)) < 1e-13
+    assert _rel(se.to_host(se.hlmz(ud, 0.7, 1.3, m)), o.hlmz(u, 0.7, 1.3, mo)) < 1e-12
+    assert _rel(se.to_host(se.lapl(ud, se.to_device(nu), m)), o.lapl(u, mo, nu)) < 1e-12
+    assert _rel(se.to_host(se.hlmz(ud, se.to_device(nu), se.to_device(nu), m)),
+                o.hlmz(u, nu, nu, mo)) < 1e-12
+    # numpy in, numpy out (host arrays round-trip through the device)
+    assert _rel(se.lapl(u, m), o.lapl(u, mo)) < 1e-12
+
+
+def test_abu_matches_oracle(se):
+    _log("ABu")
+    rng = np.random.default_rng(13)
+    u = rng.standard_normal((12, 15))
+    Br = rng.standard_normal((5, 4))
+    As = rng.standard_normal((2, 3))
+    assert _rel(se.ABu(As, Br, u), o.ABu(As, Br, u)) < 1e-14
+    assert _rel(se.ABu(None, Br, u), o.ABu(None, Br, u)) < 1e-14
+    assert _rel(se.ABu(As, None, u), o.ABu(As, None, u)) < 1e-14
+    v = rng.standard_normal((15, 10))
+    assert _rel(se.ABu(As, Br, v, trans=True), o.ABu(As.T, Br.T, v)) < 1e-14
+
+
+APPLY_CASES = [
+    # n, Ex, Ey, per, bc, rows per segment
+    (8, 70, 5, (False, False), "DDDD", 1),
+    (8, 70, 5, (False, False), "DDDD", 2),
+    (8, 70, 5, (False, False), "DNDN", 3),
+    (8, 70, 7, (True, False), "NNDD", 5),
+    (8, 33, 6, (False, True), "DDNN", 4),
+    (8, 32, 6, (True, True), "NNNN", 6),
+    (9, 4, 4, (False, False), "DDDD", 4),
+    (16, 20, 3, (False, False), "DDDD", 2),
+    (16, 17, 4, (True, True), "NNNN", 3),
+    (2, 300, 3, (False, False), "DDDD", 2),
+    (3, 90, 4, (True, False), "NNDD", 2),
+    (10, 30, 5, (False, False), "DDDD", 2),
+    (5, 1, 3, (True, True), "NNNN", 2),
+    (7, 2, 1, (True, True), "NNNN", 1),
+]
+
+
+@pytest.mark.parametrize("n,Ex,Ey,per,bc,S", APPLY_CASES)
+def test_fused_apply_matches_oracle_and_assembly_order(se, rows_env, n, Ex, Ey, per, bc, S):
+    _log(f"apply n={n} E={Ex}x{Ey} per={per} bc={bc} S={S}")
+    os.environ["SEM_AX_ROWS"] = str(S)
+    m, mo = _pair(se, n, Ex, per, "sinsin", Ey)
+    M = se.generateMask(bc, m)
+    Mo = o.generateMask(bc, mo)
+    rng = np.random.default_rng(100 + n)
+    u = rng.standard_normal(mo.x.shape)
+    nu, k = 0.9, 1.1
+    op = se.SEMOperator(nu, k, m, M)
+    y = se.to_host(op(se.to_device(u)))
+    ref = o.opLHS(u, nu, k, mo, Mo)
+    assert _rel(y, ref) < 1e-12
+    # assembly order: identical bits to (GPU element values) -> oracle QQ' -> mask
+    hl = se.to_host(se.hlmz(se.to_device(u), nu, k, m))
+    np.testing.assert_array_equal(y, Mo * o.gatherScatter_structural(hl, mo))
+    # masked nodes are exact zeros
+    assert np.all(y[Mo == 0] == 0)
+
+
+def test_fused_apply_is_deterministic(se):
+    _log("determinism")
+    m = se.Mesh(8, 8, 64, 64, deform=se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(5)
+    u = se.empty_field(*m.shape)
+    u.copy_(torch.randn(m.shape, generator=g, device="cuda", dtype=torch.float64))
+    a = op(u)
+    for _ in range(3):
+        assert torch.equal(op(u), a)
+
+
+def test_fused_apply_full_config2_size(se):
+    """256x256 elements, N=7 (4.19 M local nodes), deformed, Helmholtz: the
+    headline operator against the oracle at full size."""
+    _log("apply cfg2 size")
+    m, mo = _pair(se, 8, 256, (False, False), "sinsin")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(mo.x.shape)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    y = se.to_host(op(se.to_device(u)))
+    assert _rel(y, o.opLHS(u, 1.0, 1.0, mo, Mo)) < 1e-12
+
+
+def test_jacobi_matches_oracle(se):
+    _log("jacobi")
+    m, mo = _pair(se, 8, 5, (False, False), "sinsin", 4)
+    M = se.generateMask("DDDD", m)
+    op = se.SEMOperator(0.8, 1.7, m, M)
+    d = se.to_host(op.jacobi())
+    ref = o.jacobi_diag(mo, 0.8, 1.7, o.generateMask("DDDD", mo))
+    assert _rel(d, ref) < 1e-13
+
+
+def _cfg1(se):
+    m, mo = _pair(se, 9, 4, (False, False), "none")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    f = 2 * np.pi ** 2 * np.sin(np.pi * mo.x) * np.sin(np.pi * mo.y)
+    b = mo.gs(Mo * o.mass(f, mo))
+    return m, mo, M, Mo, b
+
+
+def test_pcg_config1_matches_oracle_and_exact_solution(se):
+    _log("pcg cfg1")
+    m, mo, M, Mo, b = _cfg1(se)
+    tol = 1e-10 * np.abs(b).max()
+    info_o = o.PCGInfo()
+    uo = o.pcg(b, lambda v: o.opLHS(v, 1.0, 0.0, mo, Mo), mult=mo.mult, tol=tol, info=info_o)
+    op = se.SEMOperator(1.0, 0.0, m, M)
+    info = se.PCGInfo()
+    u = se.to_host(se.pcg(se.to_device(b), op, mult=m.mult, tol=tol, info=info))
+    assert info.converged and abs(info.iters - info_o.iters) <= 2
+    assert _rel(u, uo) < 1e-9
+    assert np.abs(u - np.sin(np.pi * mo.x) * np.sin(np.pi * mo.y)).max() < 2e-10
+    # generic closure path of the mirror: same recursion, same answer
+    info_g = se.PCGInfo()
+    ug = se.to_host(se.pcg(se.to_device(b), lambda v: se.mask(se.gatherScatter(
+        se.hlmz(v, 1.0, 0.0, m), m), M), mult=m.mult, tol=tol, info=info_g))
+    assert abs(info_g.iters - info_o.iters) <= 2 and _rel(ug, uo) < 1e-9
+
+
+@pytest.mark.parametrize("per,bc", [((False, False), "DDDD"), ((True, False), "NNDD")])
+def test_pcg_jacobi_helmholtz_matches_oracle(se, per, bc):
+    _log(f"pcg jacobi {per} {bc}")
+    m, mo = _pair(se, 8, 12, per, "sinsin", 10)
+    M = se.generateMask(bc, m)
+    Mo = o.generateMask(bc, mo)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(mo.x.shape)
+    b = mo.gs(Mo * o.mass(f, mo))
+    tol = 1e-12 * np.abs(b).max()
+    dinv = o.jacobi_diag(mo, 1.0, 1.0, Mo)
+    info_o = o.PCGInfo()
+    uo = o.pcg(b, lambda v: o.opLHS(v, 1.0, 1.0, mo, Mo), opM=lambda r: dinv * r, mult=mo.mult,
+               tol=tol, info=info_o)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    info = se.PCGInfo()
+    u = se.to_host(se.pcg(se.to_device(b), op, opM=se.Diagonal(op.jacobi()), mult=m.mult,
+                          tol=tol, info=info))
+    # hundreds of iterations: reduction order shifts the count by ~1%
+    assert info.converged and abs(info.iters - info_o.iters) <= max(2, info_o.iters // 100)
+    assert _rel(u, uo) < 1e-10
+
+
+def test_pcg_maxiter_and_zero_rhs(se):
+    _log("pcg maxiter / zero rhs")
+    m, mo, M, Mo, b = _cfg1(se)
+    op = se.SEMOperator(1.0, 0.0, m, M)
+    info = se.PCGInfo()
+    with pytest.warns(RuntimeWarning, match="warning: res"):
+        se.pcg(se.to_device(b), op, mult=m.mult, tol=1e-30, maxiter=3, info=info)
+    assert info.iters == 3 and not info.converged
+    info2 = se.PCGInfo()
+    x = se.pcg(se.to_device(0 * b), op, mult=m.mult, tol=1e-12, info=info2)
+    assert info2.iters == 0 and float(x.abs().max()) == 0.0
+
+
+def test_f32_operator_and_mixed_solve(se):
+    _log("f32 operator + mixed")
+    m, mo = _pair(se, 8, 16, (False, False), "sinsin")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(mo.x.shape)
+    op32 = se.SEMOperator(1.0, 1.0, m, M, precision="f32")
+    y32 = se.to_host(op32(se.to_device(u)).double())
+    assert _rel(y32, o.opLHS(u, 1.0, 1.0, mo, Mo)) < 1e-5
+    f = rng.standard_normal(mo.x.shape)
+    b = mo.gs(Mo * o.mass(f, mo))
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    d = op64.jacobi()
+    tol = 1e-10 * np.abs(b).max()
+    info = se.PCGInfo()
+    x = se.to_host(se.pcg_mixed(se.to_device(b), op64, op32, d=d, tol=tol, inner_tol=1e-5,
+                                info=info))
+    assert info.converged
+    info64 = se.PCGInfo()
+    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), tol=tol, info=info64))
+    r = b - o.opLHS(x, 1.0, 1.0, mo, Mo)
+    assert np.abs(r).max() <= 1.01 * tol
+    assert _rel(x, x64) < 1e-8
+
+
+def test_grad_gradT_adjoint_dnu_match_oracle(se):
+    _log("grad/gradT/dnu")
+    m, mo = _pair(se, 7, 3, (False, False), "sinsin", 2)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(mo.x.shape)
+    lam = rng.standard_normal(mo.x.shape)
+    gx, gy = se.grad(u, m)
+    ox, oy = o.grad(u, mo)
+    assert _rel(gx, ox) < 1e-12 and _rel(gy, oy) < 1e-12
+    assert _rel(se.gradT(u, lam, m), o.gradT(u, lam, mo)) < 1e-12
+    g = se.to_host(se.adjoint_dnu(se.to_device(u), se.to_device(lam), m))
+    lx, ly = o.grad(lam, mo)
+    assert _rel(g, mo.B * (lx * ox + ly * oy)) < 1e-12
+
+
+def test_advect_and_transpose_match_oracle(se):
+    _log("advect")
+    mV, moV = _pair(se, 10, 3, (False, False), "sinsin", 2)
+    mD, moD = _pair(se, 15, 3, (False, False), "sinsin", 2)
+    rng = np.random.default_rng(10)
+    T = rng.standard_normal(moV.x.shape)
+    vx, vy = -moV.y, moV.x
+    ref = o.advect(T, vx, vy, moV, moD)
+    assert _rel(se.advect(T, vx, vy, mV, mD), ref) < 1e-12
+    w = rng.standard_normal(moV.x.shape)
+    At_w = se.advect(w, vx, vy, mV, mD, transpose=True)
+    # <w, A T> = <A' w, T>
+    lhs = np.sum(w * ref)
+    rhs = np.sum(At_w * T)
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
