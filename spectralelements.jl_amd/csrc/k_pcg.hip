@@ -4,10 +4,14 @@
 //   apply kernel (k_ax MODE 2): checks the stop test on the partials of the
 //       previous update (norm(r, Inf) <= tol, or k-1 == maxiter), forms
 //       t = <r, h>, beta = t / t_prev, u = h + beta u (h = d .* r or r),
-//       w = M .* QQ'(A u) and the partials of <u, w>;
-//   seam kernel: finishes the seam copies of w and their <u, w> partials;
-//   update kernel: alpha = t / <u, w>, x += alpha u, r -= alpha w, and the
-//       partials of <r, h> and norm(r, Inf) for the next iteration.
+//       w = M .* QQ'(A u) except at workgroup seams, and the partials of
+//       <u, w>_mult, which equal sum(u .* H) over the unassembled element
+//       outputs H because u is continuous and masked;
+//   update kernel: finishes the seam copies of w on the fly (same order as
+//       the seam kernel of the plain apply), alpha = t / <u, w>,
+//       x += alpha u, r -= alpha w, and the partials of <r, h> and
+//       norm(r, Inf) for the next iteration.
+// Two launches per iteration.
 // Partials are summed in a fixed order by every block that needs them, so
 // every block sees bit-identical scalars and the solve is deterministic.
 // The host launches iterations in growing batches and reads the done flag
@@ -47,8 +51,9 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
              const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
-             int nQ, PcgState* __restrict__ st, int k, double* __restrict__ part, int n3, int P,
-             int L, int NC, int per_x, int per_y, long n) {
+             int nQ, PcgState* __restrict__ st, int k, double* __restrict__ part, int n3,
+             SeamGeom sg, long n) {
+  const int P = sg.P, L = sg.L, NC = sg.NC, per_x = sg.per_x, per_y = sg.per_y;
   __shared__ double red[256];
   if (block_done(st, red)) return;
   const double pw = reduce_sum_array(Q, nQ, red, 256);
@@ -59,7 +64,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const int i = (int)(t % L);
     const int j = (int)(t / L);
     x[t] = x[t] + alpha * p[t];
-    const T rv = r[t] - alpha * w[t];
+    const T rv = r[t] - alpha * seam_final(w, i, j, sg);
     r[t] = rv;
     const T z = d ? d[t] * rv : rv;
     rho += (double)rv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
@@ -120,7 +125,9 @@ int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x
   const long n = (long)m->nl;
   const AxPlan pl = make_plan(g, true);
   const int n3 = (int)std::min<long>(512, std::max<long>(1, (n + 2047) / 2048));
-  const int nQ = pl.nk1 + pl.nk2;
+  const int nQ = pl.nk1;
+  SeamGeom sg{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x, g.per_y,
+              op->desc.bc_bits};
   // workspace: r, p, w | part[2][2*n3] | Q[nQ] | state
   const size_t vb = ((sizeof(T) * n + 255) / 256) * 256;
   const size_t pb = sizeof(double) * 4 * n3;
@@ -176,8 +183,7 @@ int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x
       pc.dev.k = k;
       SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
       hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(256), 0, st, x, r, p, w, d, Q, nQ,
-                         state, k, part + 2 * n3 * (k & 1), n3, g.P, g.L, g.NC, g.per_x, g.per_y,
-                         n);
+                         state, k, part + 2 * n3 * (k & 1), n3, sg, n);
     }
     SEM_HIP(hipGetLastError());
     SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));

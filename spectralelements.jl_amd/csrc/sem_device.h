@@ -93,4 +93,71 @@ __device__ __forceinline__ T mval(int i, int j, int L, int NC, int bc) {
   return mask_on(i, j, L, NC, bc) ? T(1) : T(0);
 }
 
+// Seam geometry of a fused apply launch (see AxPlan): strips of CXP local
+// rows, segments of SP local columns.
+struct SeamGeom {
+  int P, L, NC, CXP, nstrips, SP, nsegs, per_x, per_y, bc;
+};
+
+// Final value of node copy (i, j) of y = M QQ'(H) when the strip kernel left
+// the seam copies unsummed (the k_seams rules, evaluated per copy): returns
+// the stored value for copies the strip kernel finished. Every copy of a node
+// evaluates the same expression in the same order, so all copies get
+// identical bits.
+template <typename T>
+__device__ __forceinline__ T seam_final(const T* __restrict__ y, int i, int j, const SeamGeom& g) {
+  const long L = g.L;
+  const int r = i % g.CXP;
+  int iL = -1, iR = -1;
+  if (r == 0 && i > 0) {
+    iL = i - 1;
+    iR = i;
+  } else if (r == g.CXP - 1 && i < g.L - 1) {
+    iL = i;
+    iR = i + 1;
+  } else if (g.per_x && (i == 0 || i == g.L - 1)) {
+    iL = g.L - 1;
+    iR = 0;
+  }
+  if (iL >= 0) {
+    // x seam: raw values on both sides, on every y-interface too
+    const int b = j % g.P;
+    int jl = -1, ju = -1;
+    if (b == 0 && (j > 0 || g.per_y)) {
+      jl = j > 0 ? j - 1 : g.NC - 1;
+      ju = j;
+    } else if (b == g.P - 1 && (j < g.NC - 1 || g.per_y)) {
+      jl = j;
+      ju = j + 1 < g.NC ? j + 1 : 0;
+    }
+    T s;
+    if (jl >= 0) {
+      const T sl = y[iL + L * jl] + y[iR + L * jl];
+      const T su = y[iL + L * ju] + y[iR + L * ju];
+      s = sl + su;
+    } else {
+      s = y[iL + L * j] + y[iR + L * j];
+    }
+    return s * mval<T>(i, j, g.L, g.NC, g.bc);
+  }
+  // y seam: x-summed values on both sides
+  const int q = j % g.SP;
+  int jL = -1, jR = -1;
+  if (q == 0 && j > 0) {
+    jL = j - 1;
+    jR = j;
+  } else if (q == g.SP - 1 && j < g.NC - 1) {
+    jL = j;
+    jR = j + 1;
+  } else if (g.per_y && (j == 0 || j == g.NC - 1)) {
+    jL = g.NC - 1;
+    jR = 0;
+  }
+  if (jL >= 0) {
+    const T s = y[i + L * jL] + y[i + L * jR];
+    return s * mval<T>(i, j, g.L, g.NC, g.bc);
+  }
+  return y[i + L * j];
+}
+
 }  // namespace sem
