@@ -39,7 +39,18 @@ enum sem_status {
   SEM_ERR_BREAKDOWN = -5
 };
 
-enum sem_precision { SEM_F64 = 0, SEM_F32 = 1, SEM_F16_MFMA = 2, SEM_BF16_MFMA = 3 };
+/* Operator precision. The MFMA forms (nr == ns == 16) keep FP32 fields and
+ * run the Dr/Ds contractions on matrix cores with FP16 or BF16 inputs and
+ * FP32 accumulation; the X3 forms split each input into hi + lo and sum
+ * three products (lo*hi + hi*lo + hi*hi), about FP32 accuracy. */
+enum sem_precision {
+  SEM_F64 = 0,
+  SEM_F32 = 1,
+  SEM_F16_MFMA = 2,
+  SEM_BF16_MFMA = 3,
+  SEM_F16X3_MFMA = 4,
+  SEM_BF16X3_MFMA = 5
+};
 
 enum sem_deform {
   SEM_DEFORM_NONE = 0,     /* fixU, mesh.jl:6-8 */
@@ -170,7 +181,8 @@ typedef struct sem_op_desc {
 /* Builds a device operator. precision SEM_F64 keeps FP64 factors; SEM_F32
  * stores them in FP32; SEM_F16_MFMA / SEM_BF16_MFMA (nr == ns == 16) run the
  * Dr/Ds contractions as FP16/BF16-input, FP32-accumulate MFMA with FP32
- * factors and fields. */
+ * factors and fields (SEM_F16X3_MFMA / SEM_BF16X3_MFMA: split inputs, three
+ * products). */
 int sem_operator_create(sem_ctx* ctx, const sem_mesh* mesh, const sem_op_desc* desc,
                         int precision, sem_operator** out);
 int sem_operator_destroy(sem_operator* op);

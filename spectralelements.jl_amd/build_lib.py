@@ -18,7 +18,12 @@ BUILD = os.path.join(CSRC, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libsemhip.so")
 
-SOURCES = ["sem_host.cpp", "k_ax.hip", "k_misc.hip", "k_pcg.hip", "k_mixed.hip", "k_mfma.hip"]
+# (source, extra flags, object name); the fused apply is instantiated once per
+# P in its own unit so the instantiations compile in parallel
+UNITS = ([("sem_host.cpp", [], "sem_host"), ("k_ax.hip", [], "k_ax"),
+          ("k_misc.hip", [], "k_misc"), ("k_pcg.hip", [], "k_pcg"), ("k_mixed.hip", [], "k_mixed")]
+         + [("k_ax_inst.hip", [f"-DSEM_P={p}", f"-DSEM_PART={q}"], f"k_ax_p{p}_{q}")
+            for p in range(16, 1, -1) for q in (range(7) if p == 16 else range(2))])
 
 # Packed FP32 instructions are disabled: some of their forms have produced
 # wrong results on these parts; scalar FP32 code is used instead.
@@ -57,23 +62,38 @@ def gen_tables(verbose: bool = False) -> None:
         print(f"tables -> {path}")
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
+def _newest_input() -> float:
+    t = 0.0
+    for root, _, files in os.walk(CSRC):
+        if os.path.basename(root) == "_build":
+            continue
+        for f in files:
+            if f.endswith((".hip", ".cpp", ".h")):
+                t = max(t, os.path.getmtime(os.path.join(root, f)))
+    return max(t, os.path.getmtime(os.path.join(os.path.dirname(HERE), "include", "sem_abi.h")))
+
+
+def build(jobs: int = 8, verbose: bool = False, force: bool = True) -> str:
+    """Compiles every unit (force=True) or only units older than any source."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     gen_tables(verbose)
     cc = hipcc()
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs, procs = [], []
+    newest = _newest_input()
+    objs = []
     t0 = time.time()
-    pending = list(srcs)
+    pending = []
+    for s, extra, name in UNITS:
+        o = os.path.join(BUILD, name + ".o")
+        objs.append(o)
+        if force or not os.path.exists(o) or os.path.getmtime(o) < newest:
+            pending.append((s, extra, name))
     running = []
     while pending or running:
         while pending and len(running) < jobs:
-            s = pending.pop(0)
-            o = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
-            objs.append(o)
-            lang = ["-x", "hip"] if s.endswith(".hip") else ["-x", "hip"]
-            cmd = [cc, *FLAGS, *lang, "-c", os.path.join(CSRC, s), "-o", o]
+            s, extra, name = pending.pop(0)
+            o = os.path.join(BUILD, name + ".o")
+            cmd = [cc, *FLAGS, *extra, "-x", "hip", "-c", os.path.join(CSRC, s), "-o", o]
             if verbose:
                 print(" ".join(cmd), flush=True)
             running.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -106,9 +126,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only units older than the newest source")
     a = ap.parse_args()
     try:
-        build(a.jobs, a.verbose)
+        build(a.jobs, a.verbose, force=not a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -1,498 +1,18 @@
-// Fused SEM operator apply  y = M .* QQ'(nu A u + k B u)  for gfx950.
-//
-// Reference: laplace (lapl.jl:70-81), hlmz (hlmz.jl:12-19), gatherScatter
-// (gatherScatter.jl:8-16, dense QQtx*u*QQty'), mask (mask.jl:10-18), composed as
-// opLHS (diffusion.jl:36-45).
-//
-// Work decomposition (fields stay in the reference's column-major local layout,
-// L = P*Ex rows, NC = P*Ey columns):
-//  * A workgroup owns a strip of CX consecutive elements in x (CX*P local rows,
-//    one thread per row) and sweeps a segment of S element rows upward.
-//  * "Row owner": thread (e, a) holds row a of element e for all P columns b,
-//    so every global load/store is lane-contiguous along x (fully coalesced).
-//    The s-direction contractions (u Ds', ws Ds) run in registers.
-//  * "Column owner": the same thread, renamed (e, b'=a), reads column b' of
-//    element e from LDS and runs the r-direction contractions (Dr u, Dr' wr)
-//    in registers. Four LDS transposes per element row; Dr/Ds are uniform
-//    (scalar loads).
-//  * Gather-scatter inside the workgroup: x-pairs are summed through LDS, then
-//    y-pairs through a one-value register carry between consecutive element
-//    rows, so every node gets fl(fl(u1+u2)+fl(u3+u4)) with x-pairs first --
-//    the value the reference's two 0/1 GEMMs produce.
-//  * Nodes whose partner copies live in another workgroup (strip seams in x,
-//    segment seams in y, periodic wraps) are written unsummed and finished by
-//    k_seams in the same order. Every node copy is finalised exactly once.
-//  * No atomics; the result is deterministic.
-#include "sem_device.h"
-#include "_gen/gll_dtables.h"
-
-// Tuning constants: threads per strip workgroup (one per local row, so a
-// strip is SEM_AX_THREADS / P elements wide) and the largest P * sizeof(T)
-// for which the geometric factors are staged one phase early.
-#ifndef SEM_AX_THREADS
-#define SEM_AX_THREADS 256
-#endif
-#ifndef SEM_AX_EARLYG_BYTES
-#define SEM_AX_EARLYG_BYTES 80
-#endif
+// Dispatch of the fused apply over P (2..16); the kernels are instantiated
+// per P in k_ax_inst.hip.
+#include "k_ax_impl.h"
 
 namespace sem {
-namespace {
-
-template <typename T>
-struct AxP {
-  const T* __restrict__ u;
-  T* __restrict__ y;
-  const T* __restrict__ G11;
-  const T* __restrict__ G12;
-  const T* __restrict__ G22;
-  const T* __restrict__ B;
-  const T* __restrict__ Dr;   // P x P column-major: Dr(i + P*j) = Dr_ij
-  const T* __restrict__ Ds;
-  const T* __restrict__ nu_f;
-  const T* __restrict__ k_f;
-  T nu, k;
-  int has_mass;
-  int Ex, Ey, S, nstrips;
-  int per_x, per_y, bc;
-  // MODE 2 (fused PCG iteration): u is formed on load as z + beta p_old
-  // (z = d .* r or r) and written back to p; <u, y> partials go to pcg.Q
-  const T* __restrict__ r;
-  const T* __restrict__ d;
-  T* __restrict__ p;
-  PcgDev pcg;
-};
-
-// MODE 0: element-local H u (no assembly); MODE 1: assembled + masked;
-// MODE 2: one fused PCG iteration body (see AxP).
-template <typename T, int P, int CX, int MODE>
-__global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
-  constexpr int NR = CX * P;
-  constexpr int RS = NR + 1;
-  __shared__ T s0[P * RS];
-  __shared__ T s1[P * RS];
-  __shared__ double red[NR];
-
-  const int tid = threadIdx.x;
-  T beta = T(0);
-  bool first = false;
-  if (MODE == 2) {
-    if (block_done(prm.pcg.state, red)) return;
-    double t, res;
-    reduce_partials(prm.pcg.part_prev, prm.pcg.n3, red, NR, &t, &res);
-    const int k = prm.pcg.k;
-    const bool done = !(res > prm.pcg.tol) || (k - 1 == prm.pcg.maxiter);
-    if (done) {
-      if (blockIdx.x == 0 && tid == 0) {
-        prm.pcg.state->iters = k - 1;
-        prm.pcg.state->converged = res <= prm.pcg.tol ? 1 : 0;
-        prm.pcg.state->res = res;
-        prm.pcg.state->done = 1;
-      }
-      return;
-    }
-    first = (k == 1);
-    if (!first) beta = T(t / prm.pcg.state->t[(k - 1) & 1]);
-    if (blockIdx.x == 0 && tid == 0) prm.pcg.state->t[k & 1] = t;
-  }
-  double dot = 0.0;
-  T carry_u = T(0);
-  const int strip = blockIdx.x % prm.nstrips;
-  const int seg = blockIdx.x / prm.nstrips;
-  const int Ex = prm.Ex, Ey = prm.Ey;
-  const int L = P * Ex;
-  const int NC = P * Ey;
-  const int ex0 = strip * CX;
-  const int ncx = min(CX, Ex - ex0);
-  const int e = tid / P;
-  const int a = tid - e * P;
-  const bool act = e < ncx;
-  const int i = ex0 * P + tid;
-  const int ey0 = seg * prm.S;
-  const int ey1 = min(Ey, ey0 + prm.S);
-  // Dr = Ds = derivMat(gausslobatto(P)) (mesh.jl:70-74, nr == ns) is a
-  // compile-time table generated by the library's own host routine (the
-  // mesh checks bit-equality at creation), so every coefficient is an
-  // immediate the compiler rematerialises in SGPRs: no loads, no spills.
-#define Dr(ix) T(gll_d<P>(ix))
-#define Ds(ix) T(gll_d<P>(ix))
-
-  // x-edge classification of this thread's row
-  bool xseam = false, xpair = false;
-  int xside = -1;
-  if (MODE != 0 && act) {
-    const int ex = ex0 + e;
-    if (a == 0) {
-      xside = 0;
-      if (ex > 0 || prm.per_x) {
-        if (e > 0) xpair = true; else xseam = true;
-      }
-    } else if (a == P - 1) {
-      xside = 1;
-      if (ex < Ex - 1 || prm.per_x) {
-        if (e < ncx - 1) xpair = true; else xseam = true;
-      }
-    }
-  }
-
-  // Inactive threads (past the last element of a partial strip) load from a
-  // valid row so that every load is unconditional and the compiler can issue
-  // a phase's loads back to back; their results are never stored.
-  const int ic = act ? i : L - 1;
-  // G is staged in registers one phase early when it fits, so its latency
-  // hides under the C1 contraction.
-  constexpr bool kEarlyG = P * sizeof(T) <= SEM_AX_EARLYG_BYTES;
-  const bool plain = prm.nu_f == nullptr && prm.k_f == nullptr;
-
-  T carry = T(0);
-  for (int ey = ey0; ey < ey1; ++ey) {
-    const int jb = ey * P;
-    const int base = i + L * jb;
-    const int bc_ = ic + L * jb;
-    T uv[P], us[P];
-    // R1: load row a of element e, s-contraction us(a,b) = sum_k u(a,k) Ds_{b,k}
-    if (MODE == 2) {
-      T rv[P], zv[P];
-#pragma unroll
-      for (int b = 0; b < P; ++b) rv[b] = prm.r[bc_ + L * b];
-      if (prm.d) {
-#pragma unroll
-        for (int b = 0; b < P; ++b) zv[b] = prm.d[bc_ + L * b] * rv[b];
-      } else {
-#pragma unroll
-        for (int b = 0; b < P; ++b) zv[b] = rv[b];
-      }
-      if (first) {
-#pragma unroll
-        for (int b = 0; b < P; ++b) uv[b] = zv[b];
-      } else {
-#pragma unroll
-        for (int b = 0; b < P; ++b) uv[b] = zv[b] + beta * prm.p[bc_ + L * b];
-      }
-      if (act) {
-#pragma unroll
-        for (int b = 0; b < P; ++b) prm.p[base + L * b] = uv[b];
-      }
-    } else {
-#pragma unroll
-      for (int b = 0; b < P; ++b) uv[b] = prm.u[bc_ + L * b];
-    }
-#pragma unroll
-    for (int b = 0; b < P; ++b) {
-      T acc = T(0);
-#pragma unroll
-      for (int k = 0; k < P; ++k) acc += uv[k] * Ds(b + P * k);
-      us[b] = acc;
-    }
-#pragma unroll
-    for (int b = 0; b < P; ++b) s0[b * RS + tid] = uv[b];
-    T g11[kEarlyG ? P : 1], g12[kEarlyG ? P : 1], g22[kEarlyG ? P : 1];
-    if (kEarlyG) {
-#pragma unroll
-      for (int b = 0; b < (kEarlyG ? P : 1); ++b) {
-        g11[b] = prm.G11[bc_ + L * b];
-        g12[b] = prm.G12[bc_ + L * b];
-        g22[b] = prm.G22[bc_ + L * b];
-      }
-    }
-    __syncthreads();
-    // C1: column owner (e, b'=a): ur(:, b') = Dr u(:, b')
-    {
-      T v[P];
-#pragma unroll
-      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
-#pragma unroll
-      for (int a2 = 0; a2 < P; ++a2) {
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < P; ++k) acc += Dr(a2 + P * k) * v[k];
-        s1[a * RS + e * P + a2] = acc;
-      }
-    }
-    __syncthreads();
-    // R2: wr = G11 ur + G12 us, ws = G12 ur + G22 us; part2(a,b) = sum_k ws(a,k) Ds_{k,b}
-    T p2[P];
-    {
-      T wsv[P];
-#pragma unroll
-      for (int b = 0; b < P; ++b) {
-        const T ur = s1[b * RS + tid];
-        T x11, x12, x22;
-        if (kEarlyG) {
-          x11 = g11[kEarlyG ? b : 0];
-          x12 = g12[kEarlyG ? b : 0];
-          x22 = g22[kEarlyG ? b : 0];
-        } else {
-          x11 = prm.G11[bc_ + L * b];
-          x12 = prm.G12[bc_ + L * b];
-          x22 = prm.G22[bc_ + L * b];
-        }
-        const T wr = x11 * ur + x12 * us[b];
-        wsv[b] = x12 * ur + x22 * us[b];
-        s0[b * RS + tid] = wr;
-      }
-#pragma unroll
-      for (int b = 0; b < P; ++b) {
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < P; ++k) acc += wsv[k] * Ds(k + P * b);
-        p2[b] = acc;
-      }
-    }
-    T bv[P];
-    if (prm.has_mass && plain) {
-#pragma unroll
-      for (int b = 0; b < P; ++b) bv[b] = prm.B[bc_ + L * b];
-    }
-    __syncthreads();
-    // C2: column owner: part1(:, b') = Dr' wr(:, b')
-    {
-      T v[P];
-#pragma unroll
-      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
-#pragma unroll
-      for (int a2 = 0; a2 < P; ++a2) {
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < P; ++k) acc += Dr(k + P * a2) * v[k];
-        s1[a * RS + e * P + a2] = acc;
-      }
-    }
-    __syncthreads();
-    // R3: H = nu (part1 + part2) + k (B u)
-    T h[P];
-    if (plain) {
-      if (prm.has_mass) {
-#pragma unroll
-        for (int b = 0; b < P; ++b)
-          h[b] = prm.nu * (s1[b * RS + tid] + p2[b]) + prm.k * (bv[b] * uv[b]);
-      } else {
-#pragma unroll
-        for (int b = 0; b < P; ++b) h[b] = prm.nu * (s1[b * RS + tid] + p2[b]);
-      }
-    } else {
-#pragma unroll
-      for (int b = 0; b < P; ++b) {
-        const int idx = bc_ + L * b;
-        const T au = s1[b * RS + tid] + p2[b];
-        const T nuv = prm.nu_f ? prm.nu_f[idx] : prm.nu;
-        T hv = nuv * au;
-        if (prm.has_mass) {
-          const T kv = prm.k_f ? prm.k_f[idx] : prm.k;
-          hv = hv + kv * (prm.B[idx] * uv[b]);
-        }
-        h[b] = hv;
-      }
-    }
-    if (MODE == 2 && act) {
-      // <p, M QQ'(H)>_mult = sum over all local copies of p .* H for a
-      // continuous, masked p, so the partial needs no assembled values
-#pragma unroll
-      for (int b = 0; b < P; ++b) dot += (double)uv[b] * (double)h[b];
-    }
-    if (MODE == 0) {
-      if (act) {
-#pragma unroll
-        for (int b = 0; b < P; ++b) prm.y[base + L * b] = h[b];
-      }
-      continue;
-    }
-    // x-pairs inside the strip through LDS (s0 was last read in C2)
-    if (xside >= 0) {
-#pragma unroll
-      for (int b = 0; b < P; ++b) s0[(xside * CX + e) * P + b] = h[b];
-    }
-    __syncthreads();
-    if (xpair) {
-      if (xside == 1) {
-#pragma unroll
-        for (int b = 0; b < P; ++b) h[b] = h[b] + s0[(e + 1) * P + b];
-      } else {
-#pragma unroll
-        for (int b = 0; b < P; ++b) h[b] = s0[(CX + e - 1) * P + b] + h[b];
-      }
-    }
-    if (act) {
-      const int bc = prm.bc;
-      // final (masked) write of a node copy
-      auto fin = [&](int idx, int j, T val, T) { prm.y[idx] = val * mval<T>(i, j, L, NC, bc); };
-#pragma unroll
-      for (int b = 0; b < P; ++b) {
-        const int j = jb + b;
-        const int idx = base + L * b;
-        if (b == 0) {
-          const bool ysh = ey > 0 || prm.per_y;
-          if (ysh) {
-            if (ey > ey0) {
-              if (xseam) {
-                prm.y[idx] = h[0];
-                prm.y[idx - L] = carry;
-              } else {
-                const T s = carry + h[0];
-                fin(idx, j, s, uv[0]);
-                fin(idx - L, j - 1, s, carry_u);
-              }
-            } else {
-              prm.y[idx] = h[0];  // y seam: finished by k_seams
-            }
-          } else {
-            if (xseam) prm.y[idx] = h[0];
-            else fin(idx, j, h[0], uv[0]);
-          }
-        } else if (b == P - 1) {
-          const bool ysh = ey < Ey - 1 || prm.per_y;
-          if (ysh) {
-            if (ey < ey1 - 1) {
-              carry = h[P - 1];
-              carry_u = uv[P - 1];
-            } else {
-              prm.y[idx] = h[P - 1];  // y seam
-            }
-          } else {
-            if (xseam) prm.y[idx] = h[P - 1];
-            else fin(idx, j, h[P - 1], uv[P - 1]);
-          }
-        } else {
-          if (xseam) prm.y[idx] = h[b];
-          else fin(idx, j, h[b], uv[b]);
-        }
-      }
-    }
-    __syncthreads();  // s0 is rewritten by the next element row
-  }
-  if (MODE == 2) {
-    const double tot = block_sum(dot, red, NR);
-    if (tid == 0) prm.pcg.Q[blockIdx.x] = tot;
-  }
-#undef Dr
-#undef Ds
-}
-
-// Finishes the node copies the strips could not sum: x seams (strip
-// boundaries and the periodic x wrap), then y seams (segment boundaries and
-// the periodic y wrap). One item per seam row/column entry; items touch
-// disjoint copies, so there are no races.
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs,
-        int per_x, int per_y, int bc, long nxitems, long total) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  auto fin = [&](int i, int j, T s) { y[i + (long)L * j] = s * mval<T>(i, j, L, NC, bc); };
-  if (t < nxitems) {
-    const int s = (int)(t / NC);
-    const int j = (int)(t - (long)s * NC);
-    int iL, iR;
-    if (s < nstrips - 1) {
-      iR = (s + 1) * CXP;
-      iL = iR - 1;
-    } else {
-      iL = L - 1;
-      iR = 0;
-    }
-    const int b = j % P;
-    if (b == 0 && (j > 0 || per_y)) return;  // summed by the item of the row below
-    if (b == P - 1 && (j < NC - 1 || per_y)) {
-      const int jU = (j + 1 == NC) ? 0 : j + 1;
-      const T sl = y[iL + (long)L * j] + y[iR + (long)L * j];
-      const T su = y[iL + (long)L * jU] + y[iR + (long)L * jU];
-      const T sm = sl + su;
-      fin(iL, j, sm);
-      fin(iR, j, sm);
-      fin(iL, jU, sm);
-      fin(iR, jU, sm);
-    } else {
-      const T sm = y[iL + (long)L * j] + y[iR + (long)L * j];
-      fin(iL, j, sm);
-      fin(iR, j, sm);
-    }
-  } else if (t < total) {
-    const long t2 = t - nxitems;
-    const int g = (int)(t2 / L);
-    const int i = (int)(t2 - (long)g * L);
-    const int r = i % CXP;
-    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
-                        (per_x && (i == 0 || i == L - 1));
-    if (xs_row) return;
-    int jL, jR;
-    if (g < nsegs - 1) {
-      jR = (g + 1) * SP;
-      jL = jR - 1;
-    } else {
-      jL = NC - 1;
-      jR = 0;
-    }
-    const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
-    fin(i, jL, sm);
-    fin(i, jR, sm);
-  }
-}
-
-// Element rows swept per workgroup. SEM_AX_ROWS overrides the heuristic (used
-// by the tests to exercise every seam configuration).
-int rows_per_segment(const Layout& g, int cx) {
-  const char* env = getenv("SEM_AX_ROWS");
-  const int forced = env ? atoi(env) : 0;
-  if (forced > 0) return forced;
-  const int nstrips = (g.Ex + cx - 1) / cx;
-  // aim for >= 1024 workgroups (4 per CU) while keeping seams rare
-  int S = 1;
-  while (S < g.Ey && (long)nstrips * ((g.Ey + 2 * S - 1) / (2 * S)) >= 1024) S *= 2;
-  return S;
-}
-
-template <typename T, int P>
-AxP<T> make_params(const Layout& g, const ApplyArgs& a, const AxPlan& pl) {
-  AxP<T> prm{};
-  prm.u = static_cast<const T*>(a.u);
-  prm.y = static_cast<T*>(a.y);
-  prm.G11 = static_cast<const T*>(a.G11);
-  prm.G12 = static_cast<const T*>(a.G12);
-  prm.G22 = static_cast<const T*>(a.G22);
-  prm.B = static_cast<const T*>(a.B);
-  prm.Dr = static_cast<const T*>(a.Dr);
-  prm.Ds = static_cast<const T*>(a.Ds);
-  prm.nu_f = static_cast<const T*>(a.nu_f);
-  prm.k_f = static_cast<const T*>(a.k_f);
-  prm.nu = T(a.nu);
-  prm.k = T(a.k);
-  prm.has_mass = a.has_mass;
-  prm.Ex = g.Ex;
-  prm.Ey = g.Ey;
-  prm.nstrips = pl.nstrips;
-  prm.S = pl.S;
-  prm.per_x = g.per_x;
-  prm.per_y = g.per_y;
-  prm.bc = a.bc;
-  return prm;
-}
-
-template <typename T, int P>
-int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
-  constexpr int CX = SEM_AX_THREADS / P;
-  const AxPlan pl = make_plan(g, a.assemble != 0);
-  AxP<T> prm = make_params<T, P>(g, a, pl);
-  const dim3 grid(pl.nk1), block(CX * P);
-  if (pc) {
-    prm.r = static_cast<const T*>(pc->r);
-    prm.d = static_cast<const T*>(pc->d);
-    prm.p = static_cast<T*>(pc->p);
-    prm.u = nullptr;
-    prm.pcg = pc->dev;
-    hipLaunchKernelGGL((k_ax<T, P, CX, 2>), grid, block, 0, st, prm);
-  } else if (a.assemble) {
-    hipLaunchKernelGGL((k_ax<T, P, CX, 1>), grid, block, 0, st, prm);
-  } else {
-    hipLaunchKernelGGL((k_ax<T, P, CX, 0>), grid, block, 0, st, prm);
-  }
-  // In PCG mode the seam copies are finished by the update kernel instead.
-  if (a.assemble && pl.nk2 > 0 && !pc) {
-    hipLaunchKernelGGL((k_seams<T>), dim3(pl.nk2), dim3(256), 0, st, prm.y, g.L, g.NC, P, CX * P,
-                       pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, pl.nxitems,
-                       pl.total);
-  }
-  SEM_HIP(hipGetLastError());
-  return SEM_OK;
-}
+namespace ax {
+#define SEM_AX_EXTERN(P)                                                                        \
+  extern template int launch_apply_p<double, P>(hipStream_t, const Layout&, const ApplyArgs&, \
+                                                const PcgLaunch*);                            \
+  extern template int launch_apply_p<float, P>(hipStream_t, const Layout&, const ApplyArgs&,  \
+                                               const PcgLaunch*);
+SEM_AX_EXTERN(2) SEM_AX_EXTERN(3) SEM_AX_EXTERN(4) SEM_AX_EXTERN(5) SEM_AX_EXTERN(6)
+SEM_AX_EXTERN(7) SEM_AX_EXTERN(8) SEM_AX_EXTERN(9) SEM_AX_EXTERN(10) SEM_AX_EXTERN(11)
+SEM_AX_EXTERN(12) SEM_AX_EXTERN(13) SEM_AX_EXTERN(14) SEM_AX_EXTERN(15) SEM_AX_EXTERN(16)
+#undef SEM_AX_EXTERN
 
 template <typename T>
 int dispatch(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
@@ -518,13 +38,13 @@ int dispatch(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunc
   }
 }
 
-}  // namespace
+}  // namespace ax
 
 AxPlan make_plan(const Layout& g, bool assemble) {
   AxPlan pl{};
   pl.CX = SEM_AX_THREADS / g.P;
   pl.nstrips = (g.Ex + pl.CX - 1) / pl.CX;
-  pl.S = assemble ? rows_per_segment(g, pl.CX) : 1;
+  pl.S = assemble ? ax::rows_per_segment(g, pl.CX) : 1;
   pl.nsegs = (g.Ey + pl.S - 1) / pl.S;
   pl.nk1 = pl.nstrips * pl.nsegs;
   if (assemble) {
@@ -539,12 +59,12 @@ AxPlan make_plan(const Layout& g, bool assemble) {
 
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a) {
-  return dispatch<T>(st, g, a, nullptr);
+  return ax::dispatch<T>(st, g, a, nullptr);
 }
 
 template <typename T>
 int launch_apply_pcg(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch& pc) {
-  return dispatch<T>(st, g, a, &pc);
+  return ax::dispatch<T>(st, g, a, &pc);
 }
 
 template int launch_apply<double>(hipStream_t, const Layout&, const ApplyArgs&);

@@ -1,0 +1,743 @@
+// Fused SEM operator apply  y = M .* QQ'(nu A u + k B u)  for gfx950
+// (kernel templates; instantiated per P by k_ax_inst.hip, dispatched by
+// k_ax.hip).
+//
+// Reference: laplace (lapl.jl:70-81), hlmz (hlmz.jl:12-19), gatherScatter
+// (gatherScatter.jl:8-16, dense QQtx*u*QQty'), mask (mask.jl:10-18), composed as
+// opLHS (diffusion.jl:36-45).
+//
+// Work decomposition (fields stay in the reference's column-major local layout,
+// L = P*Ex rows, NC = P*Ey columns):
+//  * A workgroup owns a strip of CX consecutive elements in x (CX*P local rows,
+//    one thread per row) and sweeps a segment of S element rows upward.
+//  * "Row owner": thread (e, a) holds row a of element e for all P columns b,
+//    so every global load/store is lane-contiguous along x (fully coalesced).
+//    The s-direction contractions (u Ds', ws Ds) run in registers.
+//  * "Column owner": the same thread, renamed (e, b'=a), reads column b' of
+//    element e from LDS and runs the r-direction contractions (Dr u, Dr' wr)
+//    in registers. Four LDS transposes per element row; Dr/Ds are uniform
+//    (scalar loads).
+//  * Gather-scatter inside the workgroup: x-pairs are summed through LDS, then
+//    y-pairs through a one-value register carry between consecutive element
+//    rows, so every node gets fl(fl(u1+u2)+fl(u3+u4)) with x-pairs first --
+//    the value the reference's two 0/1 GEMMs produce.
+//  * Nodes whose partner copies live in another workgroup (strip seams in x,
+//    segment seams in y, periodic wraps) are written unsummed and finished by
+//    k_seams in the same order. Every node copy is finalised exactly once.
+//  * No atomics; the result is deterministic.
+#pragma once
+#include "sem_device.h"
+#include "_gen/gll_dtables.h"
+
+// Tuning constants: threads per strip workgroup (one per local row, so a
+// strip is SEM_AX_THREADS / P elements wide) and the largest P * sizeof(T)
+// for which the geometric factors are staged one phase early.
+#ifndef SEM_AX_THREADS
+#define SEM_AX_THREADS 256
+#endif
+#ifndef SEM_AX_EARLYG_BYTES
+#define SEM_AX_EARLYG_BYTES 80
+#endif
+
+namespace sem {
+namespace ax {
+
+template <typename T>
+struct AxP {
+  const T* __restrict__ u;
+  T* __restrict__ y;
+  const T* __restrict__ G11;
+  const T* __restrict__ G12;
+  const T* __restrict__ G22;
+  const T* __restrict__ B;
+  const T* __restrict__ Dr;   // P x P column-major: Dr(i + P*j) = Dr_ij
+  const T* __restrict__ Ds;
+  const T* __restrict__ nu_f;
+  const T* __restrict__ k_f;
+  T nu, k;
+  int has_mass;
+  int Ex, Ey, S, nstrips;
+  int per_x, per_y, bc;
+  // MODE 2 (fused PCG iteration): u is formed on load as z + beta p_old
+  // (z = d .* r or r) and written back to p; <u, y> partials go to pcg.Q
+  const T* __restrict__ r;
+  const T* __restrict__ d;
+  T* __restrict__ p;
+  PcgDev pcg;
+};
+
+// ---- MFMA contractions for 16x16 element blocks (MF != 0) ----------------
+// One v_mfma_f32_16x16x16{f16,bf16} per element contraction (K = 16); lane l
+// holds A[l%16][4(l/16)+q], B[4(l/16)+q][l%16] and D[4(l/16)+i][l%16].
+// MF = 1: FP16 inputs; 2: BF16 inputs; 3 / 4: FP16 / BF16 inputs split as
+// hi + lo with three products (lo*hi + hi*lo + hi*hi); accumulation is FP32.
+// MF = 5 / 6: FP32 / FP64 inputs on v_mfma_{f32,f64}_16x16x4 (four K-blocks;
+// lane l holds A[l%16][4kb + l/16] and B[4kb + l/16][l%16]), exact in the
+// field precision: the 16x16 contractions without per-lane D constants. The
+// FP64 form returns D interleaved: register r of lane l is D[l/16 + 4r][l%16].
+typedef _Float16 mf_h4 __attribute__((ext_vector_type(4)));
+typedef short mf_s4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+template <int MF>
+struct MfFrag {
+  using type = mf_h4;
+};
+template <>
+struct MfFrag<2> {
+  using type = mf_s4;
+};
+template <>
+struct MfFrag<4> {
+  using type = mf_s4;
+};
+
+__device__ __forceinline__ unsigned short mf_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+
+template <int MF>
+__device__ __forceinline__ void mf_pack(const float v[4], typename MfFrag<MF>::type& hi,
+                                        typename MfFrag<MF>::type& lo) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if constexpr (MF == 1 || MF == 3) {
+      const _Float16 h = (_Float16)v[q];
+      hi[q] = h;
+      if constexpr (MF == 3) lo[q] = (_Float16)(v[q] - (float)h);
+    } else {
+      const unsigned short b = mf_bf16(v[q]);
+      hi[q] = (short)b;
+      if constexpr (MF == 4) lo[q] = (short)mf_bf16(v[q] - __uint_as_float((unsigned)b << 16));
+    }
+  }
+}
+
+template <int MF>
+__device__ __forceinline__ mf_f4 mf_mma(typename MfFrag<MF>::type a, typename MfFrag<MF>::type b,
+                                        mf_f4 c) {
+  if constexpr (MF == 1 || MF == 3) return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+  else return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+// 16x16x16 product from four 16x16x4 steps: a[kb] = A[m][4kb + kg],
+// b[kb] = B[4kb + kg][n]
+__device__ __forceinline__ mf_f4 mf_contract_k4(const float a[4], const float b[4]) {
+  mf_f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb], b[kb], c, 0, 0, 0);
+  return c;
+}
+__device__ __forceinline__ mf_d4 mf_contract_k4(const double a[4], const double b[4]) {
+  mf_d4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kb], b[kb], c, 0, 0, 0);
+  return c;
+}
+
+template <int MF>
+__device__ __forceinline__ mf_f4 mf_contract(typename MfFrag<MF>::type ah,
+                                             typename MfFrag<MF>::type al,
+                                             typename MfFrag<MF>::type bh,
+                                             typename MfFrag<MF>::type bl) {
+  mf_f4 c = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (MF >= 3) {
+    c = mf_mma<MF>(al, bh, c);
+    c = mf_mma<MF>(ah, bl, c);
+  }
+  return mf_mma<MF>(ah, bh, c);
+}
+
+// MODE 0: element-local H u (no assembly); MODE 1: assembled + masked;
+// MODE 2: one fused PCG iteration body (see AxP). MF selects the MFMA
+// contraction path (T = float, P = 16 only).
+template <typename T, int P, int CX, int MODE, int MF = 0>
+__global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
+  constexpr int NR = CX * P;
+  constexpr int RS = NR + 1;
+  static_assert(MF == 0 || (P == 16 && NR == 256), "MFMA path: 16x16 blocks");
+  static_assert(MF == 0 || MF == 6 || sizeof(T) == 4, "MFMA modes 1..5 take FP32 fields");
+  static_assert(MF != 6 || sizeof(T) == 8, "MFMA mode 6 takes FP64 fields");
+  __shared__ T s0[P * RS];
+  __shared__ T s1[P * RS];
+  // x-pair exchange buffer of the MFMA path (R3 reads s0 and s1 there)
+  __shared__ T xbuf[MF ? 2 * NR : 1];
+  __shared__ double red[NR];
+  // MFMA A-operands: D and D' (Dr = Ds); hi and lo parts (MF 1..4) or the
+  // four K-blocks (MF 5, 6)
+  using Frag = typename MfFrag<(MF >= 1 && MF <= 4) ? MF : 1>::type;
+  Frag aD, aDl, aT, aTl;
+  T aDk[4], aTk[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if constexpr (MF >= 1 && MF <= 4) {
+    const int m = lane & 15, kg = lane >> 4;
+    float dv[4], tv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      dv[q] = (float)prm.Dr[m + 16 * (4 * kg + q)];
+      tv[q] = (float)prm.Dr[(4 * kg + q) + 16 * m];
+    }
+    mf_pack<MF>(dv, aD, aDl);
+    mf_pack<MF>(tv, aT, aTl);
+  } else if constexpr (MF >= 5) {
+    const int m = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      aDk[kb] = prm.Dr[m + 16 * (4 * kb + kg)];   // D[m][4kb + kg]
+      aTk[kb] = prm.Dr[(4 * kb + kg) + 16 * m];   // D'[m][4kb + kg]
+    }
+  }
+
+  const int tid = threadIdx.x;
+  T beta = T(0);
+  bool first = false;
+  if (MODE == 2) {
+    if (block_done(prm.pcg.state, red)) return;
+    double t, res;
+    reduce_partials(prm.pcg.part_prev, prm.pcg.n3, red, NR, &t, &res);
+    const int k = prm.pcg.k;
+    const bool done = !(res > prm.pcg.tol) || (k - 1 == prm.pcg.maxiter);
+    if (done) {
+      if (blockIdx.x == 0 && tid == 0) {
+        prm.pcg.state->iters = k - 1;
+        prm.pcg.state->converged = res <= prm.pcg.tol ? 1 : 0;
+        prm.pcg.state->res = res;
+        prm.pcg.state->done = 1;
+      }
+      return;
+    }
+    first = (k == 1);
+    if (!first) beta = T(t / prm.pcg.state->t[(k - 1) & 1]);
+    if (blockIdx.x == 0 && tid == 0) prm.pcg.state->t[k & 1] = t;
+  }
+  double dot = 0.0;
+  T carry_u = T(0);
+  const int strip = blockIdx.x % prm.nstrips;
+  const int seg = blockIdx.x / prm.nstrips;
+  const int Ex = prm.Ex, Ey = prm.Ey;
+  const int L = P * Ex;
+  const int NC = P * Ey;
+  const int ex0 = strip * CX;
+  const int ncx = min(CX, Ex - ex0);
+  const int e = tid / P;
+  const int a = tid - e * P;
+  const bool act = e < ncx;
+  const int i = ex0 * P + tid;
+  const int ey0 = seg * prm.S;
+  const int ey1 = min(Ey, ey0 + prm.S);
+  // Dr = Ds = derivMat(gausslobatto(P)) (mesh.jl:70-74, nr == ns) is a
+  // compile-time table generated by the library's own host routine (the
+  // mesh checks bit-equality at creation), so every coefficient is an
+  // immediate the compiler rematerialises in SGPRs: no loads, no spills.
+#define Dr(ix) T(gll_d<P>(ix))
+#define Ds(ix) T(gll_d<P>(ix))
+
+  // x-edge classification of this thread's row
+  bool xseam = false, xpair = false;
+  int xside = -1;
+  if (MODE != 0 && act) {
+    const int ex = ex0 + e;
+    if (a == 0) {
+      xside = 0;
+      if (ex > 0 || prm.per_x) {
+        if (e > 0) xpair = true; else xseam = true;
+      }
+    } else if (a == P - 1) {
+      xside = 1;
+      if (ex < Ex - 1 || prm.per_x) {
+        if (e < ncx - 1) xpair = true; else xseam = true;
+      }
+    }
+  }
+
+  // Inactive threads (past the last element of a partial strip) load from a
+  // valid row so that every load is unconditional and the compiler can issue
+  // a phase's loads back to back; their results are never stored.
+  const int ic = act ? i : L - 1;
+  // G is staged in registers one phase early when it fits, so its latency
+  // hides under the C1 contraction.
+  constexpr bool kEarlyG = P * sizeof(T) <= SEM_AX_EARLYG_BYTES;
+  const bool plain = prm.nu_f == nullptr && prm.k_f == nullptr;
+
+  T carry = T(0);
+  for (int ey = ey0; ey < ey1; ++ey) {
+    const int jb = ey * P;
+    const int base = i + L * jb;
+    const int bc_ = ic + L * jb;
+    T uv[P], us[P];
+    // R1: load row a of element e, s-contraction us(a,b) = sum_k u(a,k) Ds_{b,k}
+    if (MODE == 2) {
+      T rv[P], zv[P];
+#pragma unroll
+      for (int b = 0; b < P; ++b) rv[b] = prm.r[bc_ + L * b];
+      if (prm.d) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) zv[b] = prm.d[bc_ + L * b] * rv[b];
+      } else {
+#pragma unroll
+        for (int b = 0; b < P; ++b) zv[b] = rv[b];
+      }
+      if (first) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) uv[b] = zv[b];
+      } else {
+#pragma unroll
+        for (int b = 0; b < P; ++b) uv[b] = zv[b] + beta * prm.p[bc_ + L * b];
+      }
+      if (act) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) prm.p[base + L * b] = uv[b];
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < P; ++b) uv[b] = prm.u[bc_ + L * b];
+    }
+    if constexpr (MF == 0) {
+#pragma unroll
+      for (int b = 0; b < P; ++b) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc += uv[k] * Ds(b + P * k);
+        us[b] = acc;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < P; ++b) s0[b * RS + tid] = uv[b];
+    T g11[kEarlyG ? P : 1], g12[kEarlyG ? P : 1], g22[kEarlyG ? P : 1];
+    if (kEarlyG) {
+#pragma unroll
+      for (int b = 0; b < (kEarlyG ? P : 1); ++b) {
+        g11[b] = prm.G11[bc_ + L * b];
+        g12[b] = prm.G12[bc_ + L * b];
+        g22[b] = prm.G22[bc_ + L * b];
+      }
+    }
+    __syncthreads();
+    if constexpr (MF != 0) {
+      // M1: per element (wave wv takes elements wv, wv+4, ...):
+      // ur = D U -> s1 [b][r];  us' = D U' -> s0 [b][r] (over u, after it is read)
+      const int n = lane & 15, kg = lane >> 4;
+      for (int e2 = wv; e2 < ncx; e2 += 4) {
+        if constexpr (MF <= 4) {
+          float bu[4], bt[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            bu[q] = (float)s0[n * RS + e2 * 16 + 4 * kg + q];
+            bt[q] = (float)s0[(4 * kg + q) * RS + e2 * 16 + n];
+          }
+          Frag bh, bl, th, tl;
+          mf_pack<MF>(bu, bh, bl);
+          mf_pack<MF>(bt, th, tl);
+          const mf_f4 ur4 = mf_contract<MF>(aD, aDl, bh, bl);
+          const mf_f4 us4 = mf_contract<MF>(aD, aDl, th, tl);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            s1[n * RS + e2 * 16 + 4 * kg + q] = ur4[q];
+            s0[(4 * kg + q) * RS + e2 * 16 + n] = us4[q];
+          }
+        } else {
+          T bu[4], bt[4];
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb) {
+            bu[kb] = s0[n * RS + e2 * 16 + 4 * kb + kg];
+            bt[kb] = s0[(4 * kb + kg) * RS + e2 * 16 + n];
+          }
+          const auto ur4 = mf_contract_k4(aDk, bu);
+          const auto us4 = mf_contract_k4(aDk, bt);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = MF == 6 ? kg + 4 * q : 4 * kg + q;
+            s1[n * RS + e2 * 16 + row] = ur4[q];
+            s0[row * RS + e2 * 16 + n] = us4[q];
+          }
+        }
+      }
+    } else {
+    // C1: column owner (e, b'=a): ur(:, b') = Dr u(:, b')
+      T v[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
+#pragma unroll
+      for (int a2 = 0; a2 < P; ++a2) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc += Dr(a2 + P * k) * v[k];
+        s1[a * RS + e * P + a2] = acc;
+      }
+    }
+    __syncthreads();
+    // R2: wr = G11 ur + G12 us, ws = G12 ur + G22 us; part2(a,b) = sum_k ws(a,k) Ds_{k,b}
+    T p2[P];
+    {
+      T wsv[P];
+#pragma unroll
+      for (int b = 0; b < P; ++b) {
+        const T ur = s1[b * RS + tid];
+        T x11, x12, x22;
+        if (kEarlyG) {
+          x11 = g11[kEarlyG ? b : 0];
+          x12 = g12[kEarlyG ? b : 0];
+          x22 = g22[kEarlyG ? b : 0];
+        } else {
+          x11 = prm.G11[bc_ + L * b];
+          x12 = prm.G12[bc_ + L * b];
+          x22 = prm.G22[bc_ + L * b];
+        }
+        const T usb = MF ? s0[b * RS + tid] : us[b];
+        const T wr = x11 * ur + x12 * usb;
+        wsv[b] = x12 * ur + x22 * usb;
+        s0[b * RS + tid] = wr;
+        if constexpr (MF != 0) s1[b * RS + tid] = wsv[b];
+      }
+      if constexpr (MF == 0) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+          T acc = T(0);
+#pragma unroll
+          for (int k = 0; k < P; ++k) acc += wsv[k] * Ds(k + P * b);
+          p2[b] = acc;
+        }
+      }
+    }
+    T bv[P];
+    if (prm.has_mass && plain) {
+#pragma unroll
+      for (int b = 0; b < P; ++b) bv[b] = prm.B[bc_ + L * b];
+    }
+    __syncthreads();
+    if constexpr (MF != 0) {
+      // M2: part1 = D' wr -> s1 [b][r]; part2' = D' ws' -> s0 [b][r] (after the reads)
+      const int n = lane & 15, kg = lane >> 4;
+      for (int e2 = wv; e2 < ncx; e2 += 4) {
+        if constexpr (MF <= 4) {
+          float bw[4], bs[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            bw[q] = (float)s0[n * RS + e2 * 16 + 4 * kg + q];
+            bs[q] = (float)s1[(4 * kg + q) * RS + e2 * 16 + n];
+          }
+          Frag wh, wl, sh, sl;
+          mf_pack<MF>(bw, wh, wl);
+          mf_pack<MF>(bs, sh, sl);
+          const mf_f4 p1 = mf_contract<MF>(aT, aTl, wh, wl);
+          const mf_f4 p2t = mf_contract<MF>(aT, aTl, sh, sl);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            s1[n * RS + e2 * 16 + 4 * kg + q] = p1[q];
+            s0[(4 * kg + q) * RS + e2 * 16 + n] = p2t[q];
+          }
+        } else {
+          T bw[4], bs[4];
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb) {
+            bw[kb] = s0[n * RS + e2 * 16 + 4 * kb + kg];
+            bs[kb] = s1[(4 * kb + kg) * RS + e2 * 16 + n];
+          }
+          const auto p1 = mf_contract_k4(aTk, bw);
+          const auto p2t = mf_contract_k4(aTk, bs);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = MF == 6 ? kg + 4 * q : 4 * kg + q;
+            s1[n * RS + e2 * 16 + row] = p1[q];
+            s0[row * RS + e2 * 16 + n] = p2t[q];
+          }
+        }
+      }
+    } else {
+    // C2: column owner: part1(:, b') = Dr' wr(:, b')
+      T v[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
+#pragma unroll
+      for (int a2 = 0; a2 < P; ++a2) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc += Dr(k + P * a2) * v[k];
+        s1[a * RS + e * P + a2] = acc;
+      }
+    }
+    __syncthreads();
+    // R3: H = nu (part1 + part2) + k (B u)
+    T h[P];
+    if (plain) {
+      if (prm.has_mass) {
+#pragma unroll
+        for (int b = 0; b < P; ++b)
+          h[b] = prm.nu * (s1[b * RS + tid] + (MF ? s0[b * RS + tid] : p2[b])) +
+                 prm.k * (bv[b] * uv[b]);
+      } else {
+#pragma unroll
+        for (int b = 0; b < P; ++b)
+          h[b] = prm.nu * (s1[b * RS + tid] + (MF ? s0[b * RS + tid] : p2[b]));
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < P; ++b) {
+        const int idx = bc_ + L * b;
+        const T au = s1[b * RS + tid] + (MF ? s0[b * RS + tid] : p2[b]);
+        const T nuv = prm.nu_f ? prm.nu_f[idx] : prm.nu;
+        T hv = nuv * au;
+        if (prm.has_mass) {
+          const T kv = prm.k_f ? prm.k_f[idx] : prm.k;
+          hv = hv + kv * (prm.B[idx] * uv[b]);
+        }
+        h[b] = hv;
+      }
+    }
+    if (MODE == 2 && act) {
+      // <p, M QQ'(H)>_mult = sum over all local copies of p .* H for a
+      // continuous, masked p, so the partial needs no assembled values
+#pragma unroll
+      for (int b = 0; b < P; ++b) dot += (double)uv[b] * (double)h[b];
+    }
+    if (MODE == 0) {
+      if (act) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) prm.y[base + L * b] = h[b];
+      }
+      if constexpr (MF != 0) __syncthreads();  // s0, s1 were read in R3
+      continue;
+    }
+    // x-pairs inside the strip through LDS (s0 was last read in C2; on the
+    // MFMA path R3 reads s0 and s1, so the exchange has its own buffer)
+    T* xb = MF ? xbuf : s0;
+    if (xside >= 0) {
+#pragma unroll
+      for (int b = 0; b < P; ++b) xb[(xside * CX + e) * P + b] = h[b];
+    }
+    __syncthreads();
+    if (xpair) {
+      if (xside == 1) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) h[b] = h[b] + xb[(e + 1) * P + b];
+      } else {
+#pragma unroll
+        for (int b = 0; b < P; ++b) h[b] = xb[(CX + e - 1) * P + b] + h[b];
+      }
+    }
+    if (act) {
+      const int bc = prm.bc;
+      // final (masked) write of a node copy
+      auto fin = [&](int idx, int j, T val, T) { prm.y[idx] = val * mval<T>(i, j, L, NC, bc); };
+#pragma unroll
+      for (int b = 0; b < P; ++b) {
+        const int j = jb + b;
+        const int idx = base + L * b;
+        if (b == 0) {
+          const bool ysh = ey > 0 || prm.per_y;
+          if (ysh) {
+            if (ey > ey0) {
+              if (xseam) {
+                prm.y[idx] = h[0];
+                prm.y[idx - L] = carry;
+              } else {
+                const T s = carry + h[0];
+                fin(idx, j, s, uv[0]);
+                fin(idx - L, j - 1, s, carry_u);
+              }
+            } else {
+              prm.y[idx] = h[0];  // y seam: finished by k_seams
+            }
+          } else {
+            if (xseam) prm.y[idx] = h[0];
+            else fin(idx, j, h[0], uv[0]);
+          }
+        } else if (b == P - 1) {
+          const bool ysh = ey < Ey - 1 || prm.per_y;
+          if (ysh) {
+            if (ey < ey1 - 1) {
+              carry = h[P - 1];
+              carry_u = uv[P - 1];
+            } else {
+              prm.y[idx] = h[P - 1];  // y seam
+            }
+          } else {
+            if (xseam) prm.y[idx] = h[P - 1];
+            else fin(idx, j, h[P - 1], uv[P - 1]);
+          }
+        } else {
+          if (xseam) prm.y[idx] = h[b];
+          else fin(idx, j, h[b], uv[b]);
+        }
+      }
+    }
+    __syncthreads();  // s0 is rewritten by the next element row
+  }
+  if (MODE == 2) {
+    const double tot = block_sum(dot, red, NR);
+    if (tid == 0) prm.pcg.Q[blockIdx.x] = tot;
+  }
+#undef Dr
+#undef Ds
+}
+
+// Finishes the node copies the strips could not sum: x seams (strip
+// boundaries and the periodic x wrap), then y seams (segment boundaries and
+// the periodic y wrap). One item per seam row/column entry; items touch
+// disjoint copies, so there are no races.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs,
+        int per_x, int per_y, int bc, long nxitems, long total) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  auto fin = [&](int i, int j, T s) { y[i + (long)L * j] = s * mval<T>(i, j, L, NC, bc); };
+  if (t < nxitems) {
+    const int s = (int)(t / NC);
+    const int j = (int)(t - (long)s * NC);
+    int iL, iR;
+    if (s < nstrips - 1) {
+      iR = (s + 1) * CXP;
+      iL = iR - 1;
+    } else {
+      iL = L - 1;
+      iR = 0;
+    }
+    const int b = j % P;
+    if (b == 0 && (j > 0 || per_y)) return;  // summed by the item of the row below
+    if (b == P - 1 && (j < NC - 1 || per_y)) {
+      const int jU = (j + 1 == NC) ? 0 : j + 1;
+      const T sl = y[iL + (long)L * j] + y[iR + (long)L * j];
+      const T su = y[iL + (long)L * jU] + y[iR + (long)L * jU];
+      const T sm = sl + su;
+      fin(iL, j, sm);
+      fin(iR, j, sm);
+      fin(iL, jU, sm);
+      fin(iR, jU, sm);
+    } else {
+      const T sm = y[iL + (long)L * j] + y[iR + (long)L * j];
+      fin(iL, j, sm);
+      fin(iR, j, sm);
+    }
+  } else if (t < total) {
+    const long t2 = t - nxitems;
+    const int g = (int)(t2 / L);
+    const int i = (int)(t2 - (long)g * L);
+    const int r = i % CXP;
+    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
+                        (per_x && (i == 0 || i == L - 1));
+    if (xs_row) return;
+    int jL, jR;
+    if (g < nsegs - 1) {
+      jR = (g + 1) * SP;
+      jL = jR - 1;
+    } else {
+      jL = NC - 1;
+      jR = 0;
+    }
+    const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
+    fin(i, jL, sm);
+    fin(i, jR, sm);
+  }
+}
+
+// Element rows swept per workgroup. SEM_AX_ROWS overrides the heuristic (used
+// by the tests to exercise every seam configuration).
+inline int rows_per_segment(const Layout& g, int cx) {
+  const char* env = getenv("SEM_AX_ROWS");
+  const int forced = env ? atoi(env) : 0;
+  if (forced > 0) return forced;
+  const int nstrips = (g.Ex + cx - 1) / cx;
+  // aim for >= 1024 workgroups (4 per CU) while keeping seams rare
+  int S = 1;
+  while (S < g.Ey && (long)nstrips * ((g.Ey + 2 * S - 1) / (2 * S)) >= 1024) S *= 2;
+  return S;
+}
+
+template <typename T, int P>
+AxP<T> make_params(const Layout& g, const ApplyArgs& a, const AxPlan& pl) {
+  AxP<T> prm{};
+  prm.u = static_cast<const T*>(a.u);
+  prm.y = static_cast<T*>(a.y);
+  prm.G11 = static_cast<const T*>(a.G11);
+  prm.G12 = static_cast<const T*>(a.G12);
+  prm.G22 = static_cast<const T*>(a.G22);
+  prm.B = static_cast<const T*>(a.B);
+  prm.Dr = static_cast<const T*>(a.Dr);
+  prm.Ds = static_cast<const T*>(a.Ds);
+  prm.nu_f = static_cast<const T*>(a.nu_f);
+  prm.k_f = static_cast<const T*>(a.k_f);
+  prm.nu = T(a.nu);
+  prm.k = T(a.k);
+  prm.has_mass = a.has_mass;
+  prm.Ex = g.Ex;
+  prm.Ey = g.Ey;
+  prm.nstrips = pl.nstrips;
+  prm.S = pl.S;
+  prm.per_x = g.per_x;
+  prm.per_y = g.per_y;
+  prm.bc = a.bc;
+  return prm;
+}
+
+template <typename T, int P, int CX, int MF>
+void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mode) {
+  if (mode == 2) hipLaunchKernelGGL((k_ax<T, P, CX, 2, MF>), grid, block, 0, st, prm);
+  else if (mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, MF>), grid, block, 0, st, prm);
+  else hipLaunchKernelGGL((k_ax<T, P, CX, 0, MF>), grid, block, 0, st, prm);
+}
+
+template <typename T, int P>
+int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
+  constexpr int CX = SEM_AX_THREADS / P;
+  const AxPlan pl = make_plan(g, a.assemble != 0);
+  AxP<T> prm = make_params<T, P>(g, a, pl);
+  const dim3 grid(pl.nk1), block(CX * P);
+  int mode = a.assemble ? 1 : 0;
+  if (pc) {
+    prm.r = static_cast<const T*>(pc->r);
+    prm.d = static_cast<const T*>(pc->d);
+    prm.p = static_cast<T*>(pc->p);
+    prm.u = nullptr;
+    prm.pcg = pc->dev;
+    mode = 2;
+  }
+  // 16x16 blocks run their contractions on matrix cores in every precision
+  // (FP64 / FP32: exact MFMA forms); SEM_AX_FMA16=1 selects the FMA path.
+  int mf = a.mfma;
+  if (mf == 0 && P == 16 && CX * P == 256) {
+    const char* env = getenv("SEM_AX_FMA16");
+    if (!(env && atoi(env) == 1)) mf = sizeof(T) == 8 ? 6 : 5;
+  }
+  if (mf == 0) {
+    launch_k1<T, P, CX, 0>(st, grid, block, prm, mode);
+  } else {
+    if constexpr (P == 16 && CX * P == 256) {
+      if constexpr (sizeof(T) == 8) {
+        if (mf != 6) {
+          set_error("FP64 fields take the FP64 contraction (mode %d requested)", mf);
+          return SEM_ERR_UNSUPPORTED;
+        }
+        launch_k1<T, P, CX, 6>(st, grid, block, prm, mode);
+      } else {
+        switch (mf) {
+          case 1: launch_k1<T, P, CX, 1>(st, grid, block, prm, mode); break;
+          case 2: launch_k1<T, P, CX, 2>(st, grid, block, prm, mode); break;
+          case 3: launch_k1<T, P, CX, 3>(st, grid, block, prm, mode); break;
+          case 4: launch_k1<T, P, CX, 4>(st, grid, block, prm, mode); break;
+          case 5: launch_k1<T, P, CX, 5>(st, grid, block, prm, mode); break;
+          default:
+            set_error("unknown MFMA contraction mode %d", mf);
+            return SEM_ERR_ARG;
+        }
+      }
+    } else {
+      set_error("MFMA contractions need nr == ns == 16");
+      return SEM_ERR_UNSUPPORTED;
+    }
+  }
+  // In PCG mode the seam copies are finished by the update kernel instead.
+  if (a.assemble && pl.nk2 > 0 && !pc) {
+    hipLaunchKernelGGL((k_seams<T>), dim3(pl.nk2), dim3(256), 0, st, prm.y, g.L, g.NC, P, CX * P,
+                       pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, pl.nxitems,
+                       pl.total);
+  }
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+}  // namespace ax
+}  // namespace sem

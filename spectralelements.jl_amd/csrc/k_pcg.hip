@@ -159,6 +159,11 @@ int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x
   aa.has_mass = op->has_mass ? 1 : 0;
   aa.bc = op->desc.bc_bits;
   aa.assemble = 1;
+  aa.mfma = op->precision == SEM_F16_MFMA     ? 1
+            : op->precision == SEM_BF16_MFMA  ? 2
+            : op->precision == SEM_F16X3_MFMA ? 3
+            : op->precision == SEM_BF16X3_MFMA ? 4
+                                               : 0;
 
   SEM_HIP(hipEventRecord(ctx->ev0, st));
   SEM_HIP(hipMemsetAsync(state, 0, sizeof(PcgState), st));
@@ -244,12 +249,8 @@ int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, co
     return pcg_run_t<double>(ctx, op, static_cast<const double*>(d),
                              static_cast<const double*>(b), static_cast<double*>(x), tol, maxiter,
                              info);
-  if (op->precision == SEM_F32)
-    return pcg_run_t<float>(ctx, op, static_cast<const float*>(d), static_cast<const float*>(b),
-                            static_cast<float*>(x), tol, maxiter, info);
-  set_error("sem_pcg: operator precision %d has no fused PCG (use sem_pcg_mixed)",
-            op->precision);
-  return SEM_ERR_UNSUPPORTED;
+  return pcg_run_t<float>(ctx, op, static_cast<const float*>(d), static_cast<const float*>(b),
+                          static_cast<float*>(x), tol, maxiter, info);
 }
 
 }  // namespace sem

@@ -127,6 +127,7 @@ struct ApplyArgs {
   int has_mass;
   int bc;
   int assemble;
+  int mfma;  // 0: FMA contractions; 1..4: MFMA (FP16, BF16, FP16 x3, BF16 x3)
 };
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);

@@ -139,6 +139,16 @@ int check_square(const sem_mesh* m) {
   return SEM_OK;
 }
 
+int mfma_mode(int precision) {
+  switch (precision) {
+    case SEM_F16_MFMA: return 1;
+    case SEM_BF16_MFMA: return 2;
+    case SEM_F16X3_MFMA: return 3;
+    case SEM_BF16X3_MFMA: return 4;
+    default: return 0;
+  }
+}
+
 ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   const sem_mesh* m = op->mesh;
   ApplyArgs a{};
@@ -158,6 +168,7 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   a.has_mass = op->has_mass ? 1 : 0;
   a.bc = op->desc.assemble ? op->desc.bc_bits : 0;
   a.assemble = op->desc.assemble;
+  a.mfma = mfma_mode(op->precision);
   return a;
 }
 
@@ -514,8 +525,8 @@ int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc
                         sem_operator** out) {
   SEM_ARG(ctx && m && desc && out, "NULL argument");
   SEM_TRY(check_square(m));
-  SEM_ARG(precision >= SEM_F64 && precision <= SEM_BF16_MFMA, "bad precision %d", precision);
-  if (precision == SEM_F16_MFMA || precision == SEM_BF16_MFMA) {
+  SEM_ARG(precision >= SEM_F64 && precision <= SEM_BF16X3_MFMA, "bad precision %d", precision);
+  if (precision >= SEM_F16_MFMA) {
     if (m->nr != 16) {
       set_error("MFMA operators need nr == ns == 16 (got %d)", m->nr);
       return SEM_ERR_UNSUPPORTED;
@@ -579,13 +590,8 @@ int sem_operator_apply(sem_ctx* ctx, const sem_operator* op, const void* u, void
   SEM_ARG(ctx && op && u && y, "NULL argument");
   SEM_ARG(u != y, "sem_operator_apply: u and y must not alias");
   const ApplyArgs a = args_of(op, u, y);
-  switch (op->precision) {
-    case SEM_F64: return launch_apply<double>(ctx->stream, op->mesh->layout(), a);
-    case SEM_F32: return launch_apply<float>(ctx->stream, op->mesh->layout(), a);
-    default:
-      set_error("sem_operator_apply: precision %d not available", op->precision);
-      return SEM_ERR_UNSUPPORTED;
-  }
+  if (op->precision == SEM_F64) return launch_apply<double>(ctx->stream, op->mesh->layout(), a);
+  return launch_apply<float>(ctx->stream, op->mesh->layout(), a);
 }
 
 int sem_operator_jacobi(sem_ctx* ctx, const sem_operator* op, double* dinv) {

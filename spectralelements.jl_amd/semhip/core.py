@@ -510,7 +510,8 @@ def norm_inf(a) -> float:
 
 
 # --------------------------------------------------------- fused operator
-_PREC = {"f64": L.SEM_F64, "f32": L.SEM_F32, "f16": L.SEM_F16_MFMA, "bf16": L.SEM_BF16_MFMA}
+_PREC = {"f64": L.SEM_F64, "f32": L.SEM_F32, "f16": L.SEM_F16_MFMA, "bf16": L.SEM_BF16_MFMA,
+         "f16x3": 4, "bf16x3": 5}
 
 
 class SEMOperator:

@@ -380,3 +380,89 @@ def test_pcg_seam_configs_match_oracle(se, rows_env, n, Ex, Ey, per, bc, S, pre)
     # true residual of the returned solution
     r = b - o.opLHS(u, 1.0, k, mo, Mo)
     assert np.abs(r).max() < 10 * tol
+
+
+# relative error of one apply against the FP64 oracle, per contraction form:
+# FP16 / BF16 inputs round u, D and the fluxes to 11 / 8 significant bits;
+# the split forms carry about 22 / 16 bits.
+MFMA_TOL = {"f32": 2e-6, "f16": 5e-3, "bf16": 4e-2, "f16x3": 2e-5, "bf16x3": 2e-4}
+
+
+@pytest.mark.parametrize("prec", ["f16", "bf16", "f16x3", "bf16x3"])
+def test_mfma_operator_matches_oracle(se, rows_env, prec):
+    """N = 15 (16x16 blocks): MFMA contractions, FP32 fields, fused GS/mask."""
+    _log(f"mfma apply {prec}")
+    os.environ["SEM_AX_ROWS"] = "2"
+    m, mo = _pair(se, 16, 20, (False, False), "sinsin", 5)
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(30)
+    u = rng.standard_normal(mo.x.shape)
+    ref = o.opLHS(u, 1.0, 1.0, mo, Mo)
+    op = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+    y = se.to_host(op(se.to_device(u)).double())
+    assert _rel(y, ref) < MFMA_TOL[prec]
+    assert np.all(y[Mo == 0] == 0)
+    # assembled output is continuous: every copy of a node holds the same value
+    g = mo.l2g.reshape(-1)
+    v = y.reshape(-1)
+    first = np.full(mo.nglobal, np.nan)
+    first[g[::-1]] = v[::-1]
+    np.testing.assert_array_equal(first[g], v)
+    # element-local form (no assembly)
+    op_e = se.SEMOperator(1.0, 1.0, m, None, precision=prec, assemble=False)
+    ye = se.to_host(op_e(se.to_device(u)).double())
+    assert _rel(ye, o.hlmz(u, 1.0, 1.0, mo)) < MFMA_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16", "f16x3", "bf16x3"])
+def test_mixed_solve_reaches_fp64_tolerance(se, prec):
+    """FP64 defect correction around each low-precision inner solve reaches
+    the FP64 tolerance; the error against the FP64 solve is reported."""
+    _log(f"mixed {prec}")
+    m, mo = _pair(se, 16, 12, (False, False), "sinsin")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(31)
+    f = rng.standard_normal(mo.x.shape)
+    b = mo.gs(Mo * o.mass(f, mo))
+    tol = 1e-10 * np.abs(b).max()
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    d = op64.jacobi()
+    op_lo = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+    info = se.PCGInfo()
+    x = se.to_host(se.pcg_mixed(se.to_device(b), op64, op_lo, d=d, tol=tol, inner_tol=1e-5,
+                                max_outer=60, info=info))
+    assert info.converged, (prec, info.outer, info.res)
+    r = b - o.opLHS(x, 1.0, 1.0, mo, Mo)
+    assert np.abs(r).max() <= 1.01 * tol
+    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), tol=tol))
+    assert _rel(x, x64) < 1e-8
+
+
+@pytest.mark.parametrize("fma16", ["0", "1"])
+def test_p16_fp64_and_fp32_contraction_paths(se, rows_env, fma16):
+    """N = 15: the FP64/FP32 operators default to the exact FP64/FP32 MFMA
+    contractions; SEM_AX_FMA16=1 selects the per-lane FMA path. Both match the
+    oracle to FP64 / FP32 rounding."""
+    _log(f"p16 contraction path fma16={fma16}")
+    old = os.environ.get("SEM_AX_FMA16")
+    os.environ["SEM_AX_FMA16"] = fma16
+    try:
+        m, mo = _pair(se, 16, 17, (True, False), "sinsin", 4)
+        M = se.generateMask("NNDD", m)
+        Mo = o.generateMask("NNDD", mo)
+        rng = np.random.default_rng(33)
+        u = rng.standard_normal(mo.x.shape)
+        ref = o.opLHS(u, 0.9, 1.1, mo, Mo)
+        y64 = se.to_host(se.SEMOperator(0.9, 1.1, m, M)(se.to_device(u)))
+        assert _rel(y64, ref) < 1e-12
+        hl = se.to_host(se.hlmz(se.to_device(u), 0.9, 1.1, m))
+        np.testing.assert_array_equal(y64, Mo * o.gatherScatter_structural(hl, mo))
+        y32 = se.to_host(se.SEMOperator(0.9, 1.1, m, M, precision="f32")(se.to_device(u)).double())
+        assert _rel(y32, ref) < MFMA_TOL["f32"]
+    finally:
+        if old is None:
+            os.environ.pop("SEM_AX_FMA16", None)
+        else:
+            os.environ["SEM_AX_FMA16"] = old
