@@ -11,7 +11,8 @@ precision solve, and the CPU baseline (the oracle's reference-faithful apply
 with the dense QQ' factors of mesh.jl:81-82, timed on a bounded sample).
 
 usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-pcg]
-                       [--no-mixed] [--no-cpu] [--rotate 2]
+                       [--no-mixed] [--no-adjoint] [--no-cd] [--cd-steps 2000]
+                       [--no-cpu] [--rotate 2]
 """
 from __future__ import annotations
 
@@ -34,16 +35,8 @@ def log(msg):
 
 
 def global_normal_field(se, m, seed):
-    """f ~ N(0,1) i.i.d. per global node (column-major global order,
-    mesh.jl:87-88), scattered to every local copy so it is continuous."""
-    import numpy as np
-    import torch
-    rng = np.random.default_rng(seed)
-    g = torch.from_numpy(rng.standard_normal(m.nglobal)).cuda()
-    l2g = m.l2g.long()
-    f = se.empty_field(*m.shape)
-    f.copy_(g[l2g])
-    return f
+    """f ~ N(0,1) i.i.d. per global node, scattered to every local copy."""
+    return se.workloads.global_normal_field(m, seed)
 
 
 def bench_apply(se, args):
@@ -139,6 +132,102 @@ def bench_mixed(se):
     return out
 
 
+def bench_adjoint(se, n_dirs=20):
+    """Steady variable-coefficient diffusion -div(nu grad u) = 1, 128x128
+    elements, N=7, FP64: nu = 1 + 0.5 tanh(16 Fourier modes, seed 41), u* the
+    solution for the seed-42 coefficient, J = 1/2 sum_local B (u - u*)^2.
+    Times the gradient (forward PCG + adjoint PCG + dJ/dnu kernel) and checks
+    it against central differences (h = 1e-5) on n_dirs continuous random
+    directions (seeds 43..)."""
+    import torch
+    from semhip import workloads as w
+    m = se.Mesh(8, 8, 128, 128, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    nu = w.coefficient_field(m, 41)
+    f = torch.ones_like(nu)
+    prob = se.VarDiffusion(m, M, f)
+    info = se.PCGInfo()
+    solver = se.diffusion_solver(tol=1e-13, info=info)
+    ustar = se.linsolve(w.coefficient_field(m, 42), prob, solver).detach()
+
+    def J_of(v):
+        u = se.linsolve(v, prob, solver)
+        e = u - ustar
+        return 0.5 * torch.sum(m.B * e * e)
+
+    v = nu.clone().requires_grad_(True)
+    J_of(v).backward()                       # warm-up (operator builds, caches)
+    torch.cuda.synchronize()
+    v = nu.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    J = J_of(v)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it_fwd = info.iters
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    it_adj = info.iters
+    g = v.grad
+    worst = 0.0
+    with torch.no_grad():
+        for s in range(n_dirs):
+            d = w.continuous(w.global_normal_field(m, 43 + s), m)
+            h = 1e-5
+            fd = (float(J_of(nu + h * d)) - float(J_of(nu - h * d))) / (2 * h)
+            ad = float(torch.sum(g * d))
+            worst = max(worst, abs(fd - ad) / abs(ad))
+    return {"workload": "-div(nu grad u)=1, 128x128 elements, N=7, deformed, nu=1+0.5tanh(16 "
+                        "Fourier modes, seed 41), u* from seed 42, J=1/2 sum B (u-u*)^2",
+            "J": float(J), "forward_s": t1 - t0, "adjoint_s": t2 - t1,
+            "gradient_s": t2 - t0, "iters_forward": it_fwd, "iters_adjoint": it_adj,
+            "tol_rel": 1e-13, "fd_dirs": n_dirs, "fd_h": 1e-5,
+            "fd_max_rel_err": worst, "fd_target": 1e-6}
+
+
+def bench_cd(se, nsteps=2000, fd_dirs=1):
+    """cd2d: 64x64 elements, N=9 (dealiased p=15), v = (-y, x), nu = 1e-3,
+    Gaussian blob sigma 0.1 at (0.5, 0), BDF2/EXT2 (BDF1/EXT1 start), dt =
+    1e-3, nsteps steps by count, FP64. Times the forward run and the adjoint
+    gradient dJ/du0 of J = sum_n dt/2 sum B (u^n)^2 with sqrt(N) recompute
+    checkpointing, and checks it against central differences."""
+    import torch
+    from semhip import workloads as w
+    mV = se.Mesh(10, 10, 64, 64, (False, False), se.deform_sinsin(0.1))
+    mD = se.Mesh(15, 15, 64, 64, (False, False), se.deform_sinsin(0.1))
+    fld = se.Field("DDDD", mV, 2)
+    cdn = se.ConvectionDiffusion("cd2d", fld, -mV.y, mV.x, se.TimeStepper(0.0, nsteps * 1e-3, 1e-3, 2),
+                                 mD, nu=1e-3, tol=1e-13)
+    u0 = w.gaussian_blob(mV)
+    se.cd_objective(cdn, u0, 3)              # warm-up
+    torch.cuda.synchronize()
+    cdn.iters_total = 0
+    t0 = time.perf_counter()
+    J = se.cd_objective(cdn, u0, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    it_fwd = cdn.iters_total
+    t0 = time.perf_counter()
+    r = se.cd_gradient(cdn, u0, nsteps)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    worst = 0.0
+    for s in range(fd_dirs):
+        d = w.global_normal_field(mV, 5 + s)
+        h = 1e-3
+        fd = (se.cd_objective(cdn, u0 + h * d, nsteps) - se.cd_objective(cdn, u0 - h * d, nsteps)) / (2 * h)
+        ad = float(torch.sum(r.grad * d))
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    return {"workload": "cd2d 64x64 elements, N=9, p_d=15, v=(-y,x), nu=1e-3, dt=1e-3, "
+                        f"{nsteps} steps, blob sigma 0.1",
+            "J": J, "forward_s": t_fwd, "ms_per_step": t_fwd / nsteps * 1e3,
+            "pcg_iters_per_step": it_fwd / nsteps, "gradient_s": t_adj,
+            "gradient_over_forward": t_adj / t_fwd, "snapshots": r.snapshots,
+            "states_held": r.peak_states, "steps_forward_total": r.steps_forward,
+            "adjoint_pcg_iters": r.iters_adj, "tol_rel": 1e-13, "fd_dirs": fd_dirs,
+            "fd_max_rel_err": worst}
+
+
 def cpu_baseline(budget_s=15.0):
     """The oracle's reference-faithful apply (ABu reshape-GEMMs and the dense
     QQtx * u * QQty' of gatherScatter.jl:13) at the headline size."""
@@ -172,6 +261,9 @@ def main():
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-cd", action="store_true")
+    ap.add_argument("--cd-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -210,6 +302,14 @@ def main():
     if not args.no_mixed:
         log("mixed")
         res["mixed"] = bench_mixed(se)
+    if not args.no_adjoint:
+        log("adjoint (steady variable-coefficient)")
+        res["adjoint"] = bench_adjoint(se)
+        torch.cuda.empty_cache()
+    if not args.no_cd:
+        log("cd2d forward + checkpointed adjoint")
+        res["cd2d"] = bench_cd(se, args.cd_steps)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline()

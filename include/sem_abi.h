@@ -211,13 +211,22 @@ typedef struct sem_pcg_info {
 int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
             const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
 
-/* Mixed precision: FP64 defect correction around a low-precision inner PCG.
- * Outer loop in FP64: r = b - op64(x) (true residual), inner: op_lo d = r
- * solved by PCG to relative tolerance inner_tol in FP32, x += d, until
- * norm(r, Inf) <= tol. If plain defect correction stops contracting, the
- * remaining iterations run as FP64 flexible CG preconditioned by the inner
- * solve (mode recorded in info->outer). d64 is the FP64 Jacobi diagonal
- * (or NULL for identity). */
+/* Mixed precision: FP64 defect correction by residual replacement ("reliable
+ * updates"). One fused PCG runs with the low-precision operator op_lo on FP32
+ * vectors; its iterate is accumulated into x in FP64 and its recursive
+ * residual is replaced by the true residual b - op64(x) (FP64 operator)
+ * whenever it has dropped by the factor inner_tol since the last replacement
+ * or after 1000 inner iterations. The search direction and iteration count
+ * carry over (the Krylov space is not restarted); at each replacement the
+ * inner vectors are rescaled by a power of two so the residual is O(1). A
+ * cycle whose true residual is not finite or grew tenfold is rolled back and
+ * the Krylov space restarted; FP16/BF16-input operators restart it at every
+ * replacement (classical defect correction). Stops when norm(b - op64(x), Inf) <= tol;
+ * SEM_WARN_MAXITER after max_outer replacements, four replacements without a
+ * 10% improvement, or max_inner total inner iterations. d64: FP64 Jacobi
+ * diagonal (or NULL for identity). info->outer = replacements, info->iters =
+ * inner iterations. Environment SEM_DEBUG_MIXED=1 prints one line per cycle
+ * to stderr. */
 int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
                   const double* d64, const double* b, double* x, double tol,
                   double inner_tol, int max_outer, int max_inner, sem_pcg_info* info);
@@ -226,6 +235,19 @@ int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op
 /* sum(a .* b .* mult) with the mesh's mult (pcg.jl:45); result to host */
 int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* b,
              double* result_host);
+/* sum(a .* b .* w) over n entries (w NULL: sum(a .* b)); result to host.
+ * The local-node sums of the objectives, e.g. 1/2 sum(B .* e .* e)
+ * (semPS.jl:210-211) */
+int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
+            double* result_host);
+/* y = [QQ'] mask(sum_j coef[j] (B .* if weighted[j]) x[j]): the composition of
+ * makeRHS! (convectionDiffusion.jl:93-110, diffusion.jl:51-65): mass terms,
+ * explicit history terms, then mask and gatherScatter. nterms <= 8; x[j] are
+ * device pointers in a host array; bc_bits < 0 skips the mask; assemble = 0
+ * skips QQ'. y may alias any x[j]. */
+int sem_combine(sem_ctx* ctx, const sem_mesh* mesh, int nterms, const double* coef,
+                const double* const* x, const int* weighted, int bc_bits, int assemble,
+                double* y);
 /* norm(a, Inf); result to host */
 int sem_norm_inf(sem_ctx* ctx, int64_t n, const double* a, double* result_host);
 
@@ -241,6 +263,11 @@ int sem_grad_t(sem_ctx* ctx, const sem_mesh* mesh, const double* vx, const doubl
  * per-element kernel of the adjoint gradient (adjoint.jl:13-17) */
 int sem_adjoint_dnu(sem_ctx* ctx, const sem_mesh* mesh, const double* u,
                     const double* lam, double* g);
+/* G_ij .* nu for a variable-coefficient diffusion operator (semPS.jl:161-163,
+ * without the dealiasing interpolation): pass the outputs as the G overrides
+ * of sem_op_desc */
+int sem_vardiff_factors(sem_ctx* ctx, const sem_mesh* mesh, const double* nu, double* G11,
+                        double* G12, double* G22);
 /* advect(T, vx, vy, mshV, mshD, Jr, Js), advect.jl:45-64: dealiased
  * Js'Jr'[B_D (J vx . J grad T)] with Jr = interpMat(zr_D, zr_V) */
 int sem_advect(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD,

@@ -35,6 +35,8 @@ __all__ = [
     "gatherScatter", "gatherScatter_structural", "mask", "pcg", "PCGInfo",
     "grad", "gradT", "advect", "bdfExtK", "TimeStepper", "jacobi_diag",
     "deform_sinsin", "deform_affine01", "fixU", "local_to_global",
+    "advect_transpose", "vardiff_operator", "vardiff_gradient", "cd_forward",
+    "cd_adjoint", "cd_objective",
 ]
 
 
@@ -577,6 +579,20 @@ def advect(T, vx, vy, mshV: Mesh, mshD: Mesh, Jr=None, Js=None):
     return ABu(Js.T, Jr.T, JCu)
 
 
+def advect_transpose(w, vx, vy, mshV: Mesh, mshD: Mesh, Jr=None, Js=None):
+    """Transpose of ``advect`` as a linear map in T: the ABu pullback rule
+    (ABu.jl:39-41) applied to each factor of advect.jl:54-61, then gradT.
+    Pinned in the tests by <advect(T), w> = <T, advect_transpose(w)>."""
+    if Jr is None:
+        Jr = interpMat(mshD.zr, mshV.zr)
+    if Js is None:
+        Js = interpMat(mshD.zs, mshV.zs)
+    Jw = ABu(Js, Jr, w) * mshD.B
+    Jux = ABu(Js, Jr, vx)
+    Juy = ABu(Js, Jr, vy)
+    return gradT(ABu(Js.T, Jr.T, Jux * Jw), ABu(Js.T, Jr.T, Juy * Jw), mshV)
+
+
 def _julia_unique(t):
     out = []
     for v in t:
@@ -629,3 +645,135 @@ class TimeStepper:
         self.istep += 1
         t[0] += self.dt
         self.bdfA, self.bdfB = bdfExtK(t, k=t.size - 1)
+
+
+# --------------------------------------------------------------------------
+# steady variable-coefficient diffusion and its adjoint gradient
+# --------------------------------------------------------------------------
+def vardiff_operator(msh: Mesh, nu, M):
+    """v -> M QQ'[Dr'(nu G11 vr + nu G12 vs) + (nu G12 vr + nu G22 vs) Ds]:
+    the 12-argument lapl of lapl.jl:54-68 with nu folded into G
+    (semPS.jl:161-165; p2d_explicit.jl:105-107)."""
+    G11, G12, G22 = nu * msh.G11, nu * msh.G12, nu * msh.G22
+    return lambda v: mask(msh.gs(laplace(v, msh.Dr, msh.Ds, G11, G12, G22)), M)
+
+
+def vardiff_gradient(msh: Mesh, nu, f, ustar, M, tol_rel=1e-13):
+    """J(nu) = 1/2 sum_local B (u - u*)^2 with A(nu) u = M QQ'(B f); returns
+    (J, u, lam, dJ/dnu) where dJ/dnu = -B (grad lam . grad u) per local node
+    (the pullback of resi = rhs - lhs(u) at lam, adjoint.jl:13-17) and lam
+    solves A lam = M QQ'(B (u - u*))."""
+    A = vardiff_operator(msh, nu, M)
+    b = msh.gs(M * (msh.B * f))
+    u = pcg(b, A, mult=msh.mult, tol=tol_rel * np.abs(b).max())
+    e = u - ustar
+    J = 0.5 * np.sum(msh.B * e * e)
+    g = msh.gs(M * (msh.B * e))
+    lam = pcg(g, A, mult=msh.mult, tol=tol_rel * max(np.abs(g).max(), 1e-300))
+    lx, ly = grad(lam, msh)
+    ux, uy = grad(u, msh)
+    return J, u, lam, -msh.B * (lx * ux + ly * uy)
+
+
+# --------------------------------------------------------------------------
+# unsteady convection-diffusion (convectionDiffusion.jl) and its adjoint
+# --------------------------------------------------------------------------
+def cd_step_matrices(mV: Mesh, mD: Mesh):
+    Jr = interpMat(mD.zr, mV.zr)
+    Js = interpMat(mD.zs, mV.zs)
+    return Jr, Js
+
+
+def _cd_precond(mV: Mesh, nu, b0, M, precond):
+    """'jacobi': the inverse assembled diagonal of the step operator (keeps
+    every iterate continuous). 'reference': u ./ B ./ b0 per local copy
+    (convectionDiffusion.jl:87-91); the per-copy B makes the iterates
+    discontinuous at shared nodes, so the solve is only defined up to the
+    kernel of QQ' and the adjoint is not its exact transpose."""
+    if precond == "reference":
+        return 1.0 / (mV.B * b0)
+    return jacobi_diag(mV, nu, b0, M)
+
+
+def cd_forward(mV: Mesh, mD: Mesh, u0, vx, vy, nu, dt, nsteps, M, k=2, tol_rel=1e-12,
+               keep=False, precond="jacobi"):
+    """BDFk/EXTk convection-diffusion (convectionDiffusion.jl:76-179) with f = 0
+    and homogeneous Dirichlet data, stepped by count. Each step: histories
+    shift (mesh.jl:199-215), coefficients from bdfExtK (time.jl:31-53), RHS
+    = QQ'(M(-sum b_{1+i} B uh_i + sum a_i exH_i)), exH_i = -advect(uh_i), and
+    a PCG solve of M QQ' hlmz(u, nu, b_1) preconditioned by ``_cd_precond``.
+    Returns the states u^0..u^N if keep."""
+    Jr, Js = cd_step_matrices(mV, mD)
+    ts = TimeStepper(0.0, nsteps * dt, dt, k=k)
+    uh = [np.zeros_like(u0) for _ in range(k)]
+    u = np.array(u0, dtype=np.float64)
+    states = [u.copy()] if keep else None
+    for _ in range(nsteps):
+        for i in range(k - 1, 0, -1):
+            uh[i] = uh[i - 1]
+        uh[0] = u.copy()
+        ts.update()
+        a, b = ts.bdfA, ts.bdfB
+        rhs = np.zeros_like(u)
+        for i in range(k):
+            if b[1 + i] != 0.0:
+                rhs = rhs - b[1 + i] * mass(uh[i], mV)
+            if a[i] != 0.0:
+                rhs = rhs + a[i] * (-advect(uh[i], vx, vy, mV, mD, Jr, Js))
+        rhs = mV.gs(mask(rhs, M))
+        opL = lambda v: mask(mV.gs(hlmz(v, nu, b[0], mV)), M)
+        pre = _cd_precond(mV, nu, b[0], M, precond)
+        u = pcg(rhs, opL, opM=lambda r: pre * r, mult=mV.mult,
+                tol=tol_rel * max(np.abs(rhs).max(), 1e-300))
+        if keep:
+            states.append(u.copy())
+    return (u, states) if keep else u
+
+
+def _cd_coefficients(dt, nsteps, k):
+    """Per-step (bdfA, bdfB) of the TimeStepper (time.jl:70-112), step 1..N."""
+    ts = TimeStepper(0.0, nsteps * dt, dt, k=k)
+    out = []
+    for _ in range(nsteps):
+        ts.update()
+        out.append((np.array(ts.bdfA, dtype=np.float64), np.array(ts.bdfB, dtype=np.float64)))
+    return out
+
+
+def cd_objective(states, mV: Mesh, dt):
+    """J = sum_{n>=1} dt/2 sum_local B (u^n)^2: the time-integrated energy of
+    the trajectory, the objective the convection-diffusion adjoint differentiates."""
+    return sum(0.5 * dt * float(np.sum(mV.B * u * u)) for u in states[1:])
+
+
+def cd_adjoint(mV: Mesh, mD: Mesh, u0, vx, vy, nu, dt, nsteps, M, k=2, tol_rel=1e-12,
+               precond="jacobi"):
+    """Discrete adjoint of cd_forward for cd_objective: returns (J, dJ/du0) with
+    dJ/du0 per local node of u0 (u0 enters only through B u0 and advect(u0),
+    so every local copy has its own derivative).
+
+    Step n solves H_n u^n = M QQ'(sum_i L_{n,i} uh_i), L_{n,i} = -b_{1+i} B
+    - a_i C, C = advect. Reverse sweep: lam^n = H_n^{-1} M QQ'(dt B u^n +
+    acc_n), then acc_{n-1-i} += L_{n,i}' lam^n with C' the transpose of
+    advect; at level 0 the sum is the gradient with respect to u0."""
+    Jr, Js = cd_step_matrices(mV, mD)
+    _, states = cd_forward(mV, mD, u0, vx, vy, nu, dt, nsteps, M, k=k, tol_rel=tol_rel,
+                           keep=True, precond=precond)
+    coefs = _cd_coefficients(dt, nsteps, k)
+    J = cd_objective(states, mV, dt)
+    acc = {n: np.zeros_like(u0) for n in range(nsteps + 1)}
+    for n in range(nsteps, 0, -1):
+        a, b = coefs[n - 1]
+        mu = mV.gs(mask(dt * mV.B * states[n] + acc[n], M))
+        opL = lambda v, b0=b[0]: mask(mV.gs(hlmz(v, nu, b0, mV)), M)
+        pre = _cd_precond(mV, nu, b[0], M, precond)
+        lam = pcg(mu, opL, opM=lambda r: pre * r, mult=mV.mult,
+                  tol=tol_rel * max(np.abs(mu).max(), 1e-300))
+        Ct = advect_transpose(lam, vx, vy, mV, mD, Jr, Js)
+        for i in range(k):
+            m = n - 1 - i
+            if m < 0:
+                continue
+            if b[1 + i] != 0.0 or a[i] != 0.0:
+                acc[m] = acc[m] - b[1 + i] * (mV.B * lam) + a[i] * (-Ct)
+    return J, acc[0]

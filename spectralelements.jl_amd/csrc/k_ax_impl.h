@@ -636,14 +636,22 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
 
 // Element rows swept per workgroup. SEM_AX_ROWS overrides the heuristic (used
 // by the tests to exercise every seam configuration).
-inline int rows_per_segment(const Layout& g, int cx) {
+inline int rows_per_segment(const Layout& g, int cx, bool pcg) {
   const char* env = getenv("SEM_AX_ROWS");
   const int forced = env ? atoi(env) : 0;
   if (forced > 0) return forced;
   const int nstrips = (g.Ex + cx - 1) / cx;
-  // aim for >= 1024 workgroups (4 per CU) while keeping seams rare
+  // Plain apply: one element row per workgroup until the grid exceeds 4096
+  // workgroups (seams are finished by a separate light kernel). PCG: the
+  // update kernel finishes seams inline, so two rows per workgroup while at
+  // least 1024 workgroups remain.
+  const long target = pcg ? 1024 : 4096;
   int S = 1;
-  while (S < g.Ey && (long)nstrips * ((g.Ey + 2 * S - 1) / (2 * S)) >= 1024) S *= 2;
+  if (pcg) {
+    while (S < g.Ey && (long)nstrips * ((g.Ey + 2 * S - 1) / (2 * S)) >= target) S *= 2;
+  } else {
+    while (S < g.Ey && (long)nstrips * ((g.Ey + S - 1) / S) > target) S *= 2;
+  }
   return S;
 }
 
@@ -683,7 +691,7 @@ void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mod
 template <typename T, int P>
 int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
   constexpr int CX = SEM_AX_THREADS / P;
-  const AxPlan pl = make_plan(g, a.assemble != 0);
+  const AxPlan pl = make_plan(g, a.assemble != 0, pc != nullptr);
   AxP<T> prm = make_params<T, P>(g, a, pl);
   const dim3 grid(pl.nk1), block(CX * P);
   int mode = a.assemble ? 1 : 0;

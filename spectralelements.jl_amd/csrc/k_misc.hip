@@ -268,6 +268,53 @@ __global__ void __launch_bounds__(256) k_final(const double* part, int n, int op
   if (threadIdx.x == 0) out[0] = r;
 }
 
+// ---- right-hand-side composition and weighted reductions
+// y = mask(sum_j c_j (B if weighted_j) x_j): the pointwise half of makeRHS!
+// (convectionDiffusion.jl:93-110, diffusion.jl:51-65); y may alias any x_j
+// (each thread reads its node of every term before writing it).
+__global__ void k_combine(CombineTerms terms, const double* __restrict__ B, double* y, int L,
+                          int NC, int bc, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double bt = B[t];
+  double acc = 0.0;
+  for (int j = 0; j < terms.n; ++j) {
+    const double v = terms.x[j][t];
+    acc += terms.c[j] * (terms.w[j] ? bt * v : v);
+  }
+  if (bc >= 0) {
+    const int i = (int)(t % L);
+    const int jj = (int)(t / L);
+    acc *= mask_on(i, jj, L, NC, bc) ? 1.0 : 0.0;
+  }
+  y[t] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+k_dot_part(const double* __restrict__ a, const double* __restrict__ b,
+           const double* __restrict__ w, long n, double* part) {
+  __shared__ double sh[256];
+  double acc = 0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    acc += w ? a[t] * b[t] * w[t] : a[t] * b[t];
+  const double s = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// nu folded into the geometric factors: G_ij <- nu .* G_ij (semPS.jl:161-163)
+__global__ void k_vardiff(const double* __restrict__ nu, const double* __restrict__ G11,
+                          const double* __restrict__ G12, const double* __restrict__ G22,
+                          double* __restrict__ o11, double* __restrict__ o12,
+                          double* __restrict__ o22, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double v = nu[t];
+  o11[t] = v * G11[t];
+  o12[t] = v * G12[t];
+  o22[t] = v * G22[t];
+}
+
 // ---- gradient (grad.jl:15-34), its transpose, and the dJ/dnu kernel
 __global__ void k_grad(const double* __restrict__ u, const double* __restrict__ Dr,
                        const double* __restrict__ Ds, const double* __restrict__ rx,
@@ -610,6 +657,32 @@ int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* 
   hipLaunchKernelGGL(k_wdot_part, dim3(nparts), dim3(256), 0, st, a, b, g.P, g.L, g.NC, g.per_x,
                      g.per_y, n, part);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_combine(hipStream_t st, const sem_mesh* m, const CombineTerms& terms, int bits,
+                   double* y) {
+  const long n = (long)m->nl;
+  hipLaunchKernelGGL(k_combine, dim3(nblk(n)), dim3(256), 0, st, terms, m->f[SEM_B], y, m->L,
+                     m->NC, bits, n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
+               double* part, int nparts, double* out) {
+  hipLaunchKernelGGL(k_dot_part, dim3(nparts), dim3(256), 0, st, a, b, w, (long)n, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_vardiff(hipStream_t st, const sem_mesh* m, const double* nu, double* G11, double* G12,
+                   double* G22) {
+  const long n = (long)m->nl;
+  hipLaunchKernelGGL(k_vardiff, dim3(nblk(n)), dim3(256), 0, st, nu, m->f[SEM_G11],
+                     m->f[SEM_G12], m->f[SEM_G22], G11, G12, G22, n);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

@@ -16,6 +16,10 @@
 // every block sees bit-identical scalars and the solve is deterministic.
 // The host launches iterations in growing batches and reads the done flag
 // between batches; iterations launched after convergence exit at once.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
 #include "sem_device.h"
 
 namespace sem {
@@ -117,93 +121,169 @@ __global__ void k_inv_masked(const double* __restrict__ d, double* __restrict__ 
 
 inline int nblk(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
+// Partials of <r, h> and norm(r, Inf) of the current residual (after a
+// residual replacement).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pcg_partials(const T* __restrict__ r, const T* __restrict__ d, int P, int L, int NC, int per_x,
+               int per_y, long n, double* __restrict__ part, int n3) {
+  __shared__ double red[256];
+  double rho = 0.0, mx = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % L);
+    const int j = (int)(t / L);
+    const T rv = r[t];
+    const T z = d ? d[t] * rv : rv;
+    rho += (double)rv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
+    mx = nanmax(mx, fabs((double)rv));
+  }
+  rho = block_sum(rho, red, 256);
+  mx = block_max(mx, red, 256);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rho;
+    part[n3 + blockIdx.x] = mx;
+  }
+}
+
+// A resumable fused PCG: init from b, run iterations until the stop test
+// fires, optionally replace the residual and continue with the same search
+// direction and iteration count (used by the mixed-precision solver).
+template <typename T>
+struct PcgRun {
+  sem_ctx* ctx = nullptr;
+  const sem_operator* op = nullptr;
+  const T* d = nullptr;
+  Layout g{};
+  long n = 0;
+  AxPlan pl{};
+  int n3 = 0, nQ = 0;
+  SeamGeom sg{};
+  ApplyArgs aa{};
+  T *r = nullptr, *p = nullptr, *w = nullptr;
+  double *part = nullptr, *Q = nullptr;
+  PcgState* state = nullptr;
+  PcgState* hs = nullptr;
+
+  int setup(sem_ctx* c, const sem_operator* o, const T* dd) {
+    ctx = c;
+    op = o;
+    d = dd;
+    const sem_mesh* m = op->mesh;
+    g = m->layout();
+    n = (long)m->nl;
+    pl = make_plan(g, true, true);
+    n3 = (int)std::min<long>(512, std::max<long>(1, (n + 2047) / 2048));
+    nQ = pl.nk1;
+    sg = SeamGeom{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x,
+                  g.per_y, op->desc.bc_bits};
+    // workspace: r, p, w | part[2][2*n3] | Q[nQ] | state
+    const size_t vb = ((sizeof(T) * n + 255) / 256) * 256;
+    const size_t pb = sizeof(double) * 4 * n3;
+    const size_t qb = ((sizeof(double) * nQ + 255) / 256) * 256;
+    SEM_TRY(ensure_ws(ctx, 3 * vb + pb + qb + 256));
+    char* w0 = static_cast<char*>(ctx->ws);
+    r = reinterpret_cast<T*>(w0);
+    p = reinterpret_cast<T*>(w0 + vb);
+    w = reinterpret_cast<T*>(w0 + 2 * vb);
+    part = reinterpret_cast<double*>(w0 + 3 * vb);
+    Q = reinterpret_cast<double*>(w0 + 3 * vb + pb);
+    state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb);
+    SEM_TRY(ensure_pinned(ctx, 128));  // PcgState, then a scalar slot at byte 64
+    hs = static_cast<PcgState*>(ctx->pinned);
+    aa.u = nullptr;
+    aa.y = w;
+    aa.G11 = op->G11;
+    aa.G12 = op->G12;
+    aa.G22 = op->G22;
+    aa.B = op->B;
+    aa.Dr = sizeof(T) == 8 ? (const void*)m->Dr : (const void*)m->Dr32;
+    aa.Ds = sizeof(T) == 8 ? (const void*)m->Ds : (const void*)m->Ds32;
+    aa.nu_f = op->nu_f;
+    aa.k_f = op->k_f;
+    aa.nu = op->desc.nu;
+    aa.k = op->desc.k;
+    aa.has_mass = op->has_mass ? 1 : 0;
+    aa.bc = op->desc.bc_bits;
+    aa.assemble = 1;
+    aa.mfma = op->precision == SEM_F16_MFMA     ? 1
+              : op->precision == SEM_BF16_MFMA  ? 2
+              : op->precision == SEM_F16X3_MFMA ? 3
+              : op->precision == SEM_BF16X3_MFMA ? 4
+                                                 : 0;
+    return SEM_OK;
+  }
+
+  // x = 0, r = b, partials of r for iteration 1, state cleared
+  int init(const T* b, T* x) {
+    hipStream_t st = ctx->stream;
+    SEM_HIP(hipMemsetAsync(state, 0, sizeof(PcgState), st));
+    hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(256), 0, st, b, x, r, d, g.P, g.L, g.NC,
+                       g.per_x, g.per_y, n, part, n3);
+    SEM_HIP(hipGetLastError());
+    return SEM_OK;
+  }
+
+  // after r was replaced: partials of r for iteration k, done flag cleared
+  int reprime(int k) {
+    hipStream_t st = ctx->stream;
+    SEM_HIP(hipMemsetAsync(&state->done, 0, sizeof(int), st));
+    hipLaunchKernelGGL((k_pcg_partials<T>), dim3(n3), dim3(256), 0, st, r, d, g.P, g.L, g.NC,
+                       g.per_x, g.per_y, n, part + 2 * n3 * ((k - 1) & 1), n3);
+    SEM_HIP(hipGetLastError());
+    return SEM_OK;
+  }
+
+  // launches iterations k_start.. until the stop test fires (norm(r, Inf) <=
+  // tol or k - 1 == maxiter); on return hs holds the final state
+  int run(T* x, double tol, int maxiter, int k_start) {
+    hipStream_t st = ctx->stream;
+    int k = k_start, batch = 4;
+    const long kmax = (long)maxiter + 1;
+    for (;;) {
+      for (int q = 0; q < batch && k <= kmax; ++q, ++k) {
+        PcgLaunch pc{};
+        pc.r = r;
+        pc.d = d;
+        pc.p = p;
+        pc.dev.part_prev = part + 2 * n3 * ((k - 1) & 1);
+        pc.dev.n3 = n3;
+        pc.dev.Q = Q;
+        pc.dev.state = state;
+        pc.dev.tol = tol;
+        pc.dev.maxiter = maxiter;
+        pc.dev.k = k;
+        SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
+        hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(256), 0, st, x, r, p, w, d, Q, nQ,
+                           state, k, part + 2 * n3 * (k & 1), n3, sg, n);
+      }
+      SEM_HIP(hipGetLastError());
+      SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+      SEM_HIP(hipStreamSynchronize(st));
+      if (hs->done) return SEM_OK;
+      if (k > kmax) {
+        set_error("pcg: iteration limit passed without the stop test firing");
+        return SEM_ERR_BREAKDOWN;
+      }
+      batch = std::min(batch * 2, 64);
+    }
+  }
+};
+
 template <typename T>
 int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x, double tol,
               int maxiter, sem_pcg_info* info) {
-  const sem_mesh* m = op->mesh;
-  const Layout g = m->layout();
-  const long n = (long)m->nl;
-  const AxPlan pl = make_plan(g, true);
-  const int n3 = (int)std::min<long>(512, std::max<long>(1, (n + 2047) / 2048));
-  const int nQ = pl.nk1;
-  SeamGeom sg{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x, g.per_y,
-              op->desc.bc_bits};
-  // workspace: r, p, w | part[2][2*n3] | Q[nQ] | state
-  const size_t vb = ((sizeof(T) * n + 255) / 256) * 256;
-  const size_t pb = sizeof(double) * 4 * n3;
-  const size_t qb = ((sizeof(double) * nQ + 255) / 256) * 256;
-  SEM_TRY(ensure_ws(ctx, 3 * vb + pb + qb + 256));
-  char* w0 = static_cast<char*>(ctx->ws);
-  T* r = reinterpret_cast<T*>(w0);
-  T* p = reinterpret_cast<T*>(w0 + vb);
-  T* w = reinterpret_cast<T*>(w0 + 2 * vb);
-  double* part = reinterpret_cast<double*>(w0 + 3 * vb);
-  double* Q = reinterpret_cast<double*>(w0 + 3 * vb + pb);
-  PcgState* state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb);
-  SEM_TRY(ensure_pinned(ctx, sizeof(PcgState)));
+  PcgRun<T> R;
+  SEM_TRY(R.setup(ctx, op, d));
   hipStream_t st = ctx->stream;
-
-  ApplyArgs aa{};
-  aa.u = nullptr;
-  aa.y = w;
-  aa.G11 = op->G11;
-  aa.G12 = op->G12;
-  aa.G22 = op->G22;
-  aa.B = op->B;
-  aa.Dr = sizeof(T) == 8 ? (const void*)m->Dr : (const void*)m->Dr32;
-  aa.Ds = sizeof(T) == 8 ? (const void*)m->Ds : (const void*)m->Ds32;
-  aa.nu_f = op->nu_f;
-  aa.k_f = op->k_f;
-  aa.nu = op->desc.nu;
-  aa.k = op->desc.k;
-  aa.has_mass = op->has_mass ? 1 : 0;
-  aa.bc = op->desc.bc_bits;
-  aa.assemble = 1;
-  aa.mfma = op->precision == SEM_F16_MFMA     ? 1
-            : op->precision == SEM_BF16_MFMA  ? 2
-            : op->precision == SEM_F16X3_MFMA ? 3
-            : op->precision == SEM_BF16X3_MFMA ? 4
-                                               : 0;
-
   SEM_HIP(hipEventRecord(ctx->ev0, st));
-  SEM_HIP(hipMemsetAsync(state, 0, sizeof(PcgState), st));
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(256), 0, st, b, x, r, d, g.P, g.L, g.NC,
-                     g.per_x, g.per_y, n, part, n3);
-  SEM_HIP(hipGetLastError());
-  PcgState* hs = static_cast<PcgState*>(ctx->pinned);
-  int k = 1, batch = 4;
-  const long kmax = (long)maxiter + 1;
-  for (;;) {
-    for (int q = 0; q < batch && k <= kmax; ++q, ++k) {
-      PcgLaunch pc{};
-      pc.r = r;
-      pc.d = d;
-      pc.p = p;
-      pc.dev.part_prev = part + 2 * n3 * ((k - 1) & 1);
-      pc.dev.n3 = n3;
-      pc.dev.Q = Q;
-      pc.dev.state = state;
-      pc.dev.tol = tol;
-      pc.dev.maxiter = maxiter;
-      pc.dev.k = k;
-      SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
-      hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(256), 0, st, x, r, p, w, d, Q, nQ,
-                         state, k, part + 2 * n3 * (k & 1), n3, sg, n);
-    }
-    SEM_HIP(hipGetLastError());
-    SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
-    SEM_HIP(hipStreamSynchronize(st));
-    if (hs->done) break;
-    if (k > kmax) {
-      set_error("pcg: iteration limit passed without the stop test firing");
-      return SEM_ERR_BREAKDOWN;
-    }
-    batch = std::min(batch * 2, 64);
-  }
+  SEM_TRY(R.init(b, x));
+  SEM_TRY(R.run(x, tol, maxiter, 1));
   SEM_HIP(hipEventRecord(ctx->ev1, st));
   SEM_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
   SEM_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  const PcgState* hs = R.hs;
   if (info) {
     info->iters = hs->iters;
     info->converged = hs->converged;
@@ -217,6 +297,195 @@ int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x
     return SEM_ERR_NONFINITE;
   }
   return hs->converged ? SEM_OK : SEM_WARN_MAXITER;
+}
+
+__global__ void k_resid(const double* __restrict__ b, double* __restrict__ r, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) r[t] = b[t] - r[t];
+}
+
+__global__ void k_to_f32(const double* __restrict__ r, float* __restrict__ r32, double s,
+                         long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) r32[t] = (float)(r[t] * s);
+}
+
+__global__ void k_accum_zero(double* __restrict__ x, float* __restrict__ e, double s, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    x[t] = x[t] + s * (double)e[t];
+    e[t] = 0.f;
+  }
+}
+
+// p *= f and the stored <r, z> of iteration k-1 *= f^2 (f a power of two, so
+// both are exact): the inner recurrence continues on a rescaled residual.
+__global__ void k_rescale(float* __restrict__ p, float f, PcgState* state, int k, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = p[t] * f;
+  if (t == 0) state->t[(k - 1) & 1] = state->t[(k - 1) & 1] * ((double)f * (double)f);
+}
+
+// power of two nearest below v (v > 0, finite)
+double pow2_near(double v) { return std::ldexp(1.0, std::ilogb(v)); }
+
+// Mixed precision with FP64 residual replacement ("reliable updates"): one
+// FP32 fused PCG whose iterate is accumulated into x in FP64 and whose
+// recursive residual is replaced by the true residual b - A64 x (fused FP64
+// operator) whenever it has dropped by the factor delta since the last
+// replacement, or after kChunk inner iterations. The search direction and the
+// iteration count carry over, so the Krylov space is not restarted. At every
+// replacement the inner vectors are rescaled by a power of two so that the
+// residual is O(1) (FP16 contraction inputs stay in their exponent range).
+// A cycle whose true residual is not finite or grew tenfold is rolled back
+// and the Krylov space restarted from the last accepted x. Operators with
+// FP16/BF16 contraction inputs restart the Krylov space at every replacement
+// (classical defect correction): their operator error is too large for the
+// carried search direction to stay useful. Stops when the
+// true FP64 residual meets tol; gives up after max_cycles replacements, four
+// without a 10% improvement, or max_inner inner iterations in total.
+constexpr int kChunk = 1000;
+
+int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
+              const double* d64, const double* b, double* x, double tol, double delta,
+              int max_cycles, int max_inner, sem_pcg_info* info) {
+  const sem_mesh* m = op64->mesh;
+  const long n = (long)m->nl;
+  hipStream_t st = ctx->stream;
+  double *r64 = nullptr, *xb = nullptr, *nrm = nullptr;
+  float *x32 = nullptr, *d32 = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(r64);
+    (void)hipFree(xb);
+    (void)hipFree(x32);
+    (void)hipFree(d32);
+    (void)hipFree(nrm);
+  };
+  if (hipMalloc(&r64, sizeof(double) * n) != hipSuccess ||
+      hipMalloc(&xb, sizeof(double) * n) != hipSuccess ||
+      hipMalloc(&x32, sizeof(float) * n) != hipSuccess ||
+      hipMalloc(&nrm, sizeof(double) * 260) != hipSuccess ||
+      (d64 && hipMalloc(&d32, sizeof(float) * n) != hipSuccess)) {
+    set_error("sem_pcg_mixed: hipMalloc failed");
+    cleanup();
+    return SEM_ERR_HIP;
+  }
+  int s = SEM_OK;
+  if (d64) s = launch_convert_f64_f32(st, n, d64, d32);
+  PcgRun<float> R;
+  if (s == SEM_OK) s = R.setup(ctx, op_lo, d32);
+  if (s != SEM_OK) {
+    cleanup();
+    return s;
+  }
+  const bool debug = std::getenv("SEM_DEBUG_MIXED") != nullptr;
+  const bool reliable = op_lo->precision == SEM_F32;
+  hipEvent_t e0, e1;
+  SEM_HIP(hipEventCreate(&e0));
+  SEM_HIP(hipEventCreate(&e1));
+  SEM_HIP(hipEventRecord(e0, st));
+  SEM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  SEM_HIP(hipMemsetAsync(x32, 0, sizeof(float) * n, st));
+  SEM_HIP(hipMemcpyAsync(r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  double* hres = reinterpret_cast<double*>(static_cast<char*>(ctx->pinned) + 64);
+  auto true_norm = [&](double* out) -> int {
+    SEM_TRY(launch_norm_inf(st, n, r64, nrm, 256, nrm + 256));
+    SEM_HIP(hipMemcpyAsync(hres, nrm + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+    SEM_HIP(hipStreamSynchronize(st));
+    *out = *hres;
+    return SEM_OK;
+  };
+  auto true_residual = [&](double* out) -> int {
+    SEM_TRY(sem_operator_apply(ctx, op64, x, r64));
+    hipLaunchKernelGGL(k_resid, dim3(nblk(n)), dim3(256), 0, st, b, r64, n);
+    return true_norm(out);
+  };
+  double res = 0.0;
+  if ((s = true_norm(&res)) != SEM_OK) {
+    cleanup();
+    return s;
+  }
+  int cycles = 0, stall = 0, k = 1, total = 0, restarts = 0, status = SEM_WARN_MAXITER;
+  double best = res, scale = 1.0;
+  if (!(res > tol)) status = (res <= tol) ? SEM_OK : SEM_ERR_NONFINITE;
+  // (re)starts the inner solve from the residual in r64 (norm res)
+  auto start_inner = [&]() -> int {
+    scale = pow2_near(res);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, st, r64, R.r, 1.0 / scale, n);
+    SEM_HIP(hipMemsetAsync(R.state, 0, sizeof(PcgState), st));
+    k = 1;
+    return R.reprime(1);
+  };
+  if (status == SEM_WARN_MAXITER) {
+    s = start_inner();
+    for (;;) {
+      if (s != SEM_OK) break;
+      const double tol_c = std::max(delta * res, 0.5 * tol) / scale;
+      const int k0 = k;
+      s = R.run(x32, tol_c, k0 - 1 + std::min(kChunk, std::max(1, max_inner - total)), k0);
+      if (s != SEM_OK) break;
+      k = R.hs->iters + 1;
+      total += k - k0;
+      const double inner_res = R.hs->res;
+      SEM_HIP(hipMemcpyAsync(xb, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_accum_zero, dim3(nblk(n)), dim3(256), 0, st, x, x32, scale, n);
+      const double res_prev = res;
+      if ((s = true_residual(&res)) != SEM_OK) break;
+      ++cycles;
+      if (debug)
+        std::fprintf(stderr, "[mixed] cycle %d: inner %d its (total %d), inner res %.3e, "
+                     "true res %.3e\n", cycles, k - k0, total, inner_res * scale, res);
+      if (res <= tol) {
+        status = SEM_OK;
+        break;
+      }
+      const bool bad = !(res < 10.0 * res_prev) || !std::isfinite(inner_res);
+      if (bad) {
+        // roll back to the last accepted iterate, restart the Krylov space
+        SEM_HIP(hipMemcpyAsync(x, xb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if ((s = true_residual(&res)) != SEM_OK) break;
+        ++restarts;
+      }
+      stall = (res < 0.9 * best) ? 0 : stall + 1;
+      best = std::min(best, res);
+      if (cycles >= max_cycles || stall >= 4 || total >= max_inner) break;
+      if (bad || !reliable) {
+        s = start_inner();
+      } else {
+        // replace the recursive residual; rescale p and <r, z> to match
+        const double ns = pow2_near(res);
+        hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, st, r64, R.r, 1.0 / ns, n);
+        hipLaunchKernelGGL(k_rescale, dim3(nblk(n)), dim3(256), 0, st, R.p,
+                           (float)(scale / ns), R.state, k, n);
+        SEM_HIP(hipGetLastError());
+        scale = ns;
+        s = R.reprime(k);
+      }
+    }
+  }
+  SEM_HIP(hipEventRecord(e1, st));
+  SEM_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  if (s != SEM_OK) return s;
+  if (!std::isfinite(res)) status = SEM_ERR_NONFINITE;
+  if (debug)
+    std::fprintf(stderr, "[mixed] done: %d cycles, %d restarts, %d inner its, res %.3e\n",
+                 cycles, restarts, total, res);
+  if (info) {
+    info->iters = total;
+    info->inner_total = total;
+    info->outer = cycles;
+    info->res = res;
+    info->converged = status == SEM_OK;
+    info->elapsed_ms = ms;
+  }
+  if (status == SEM_ERR_NONFINITE) set_error("sem_pcg_mixed: residual is not finite");
+  return status;
 }
 
 }  // namespace
@@ -252,5 +521,25 @@ int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, co
   return pcg_run_t<float>(ctx, op, static_cast<const float*>(d), static_cast<const float*>(b),
                           static_cast<float*>(x), tol, maxiter, info);
 }
+
+}  // namespace sem
+
+using namespace sem;
+
+extern "C" int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
+                             const double* d64, const double* b, double* x, double tol,
+                             double inner_tol, int max_outer, int max_inner,
+                             sem_pcg_info* info) {
+  SEM_ARG(ctx && op64 && op_lo && b && x, "NULL argument");
+  SEM_ARG(op64->precision == SEM_F64, "sem_pcg_mixed: op64 must be SEM_F64");
+  SEM_ARG(op_lo->precision != SEM_F64, "sem_pcg_mixed: op_lo must be a low-precision operator");
+  SEM_ARG(op64->mesh == op_lo->mesh, "sem_pcg_mixed: operators on different meshes");
+  SEM_ARG(op64->desc.assemble && op_lo->desc.assemble, "sem_pcg_mixed: operators must assemble");
+  SEM_ARG(max_outer >= 1 && max_inner >= 1 && inner_tol > 0 && inner_tol < 1 && tol >= 0,
+          "sem_pcg_mixed: bad limits");
+  return mixed_run(ctx, op64, op_lo, d64, b, x, tol, inner_tol, max_outer, max_inner, info);
+}
+
+namespace sem {
 
 }  // namespace sem

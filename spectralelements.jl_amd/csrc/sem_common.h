@@ -159,7 +159,7 @@ struct AxPlan {
   int CX, nstrips, S, nsegs, nk1, nk2;
   long nxitems, total;
 };
-AxPlan make_plan(const Layout& g, bool assemble);
+AxPlan make_plan(const Layout& g, bool assemble, bool pcg);
 
 struct PcgLaunch {
   const void* r;  // recursive residual
@@ -169,6 +169,21 @@ struct PcgLaunch {
 };
 template <typename T>
 int launch_apply_pcg(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch& pc);
+
+// Terms of a pointwise combination sum_j c_j (B if w_j) x_j, passed by value.
+constexpr int kMaxTerms = 8;
+struct CombineTerms {
+  const double* x[kMaxTerms];
+  double c[kMaxTerms];
+  int w[kMaxTerms];
+  int n;
+};
+int launch_combine(hipStream_t st, const sem_mesh* m, const CombineTerms& terms, int bits,
+                   double* y);
+int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
+               double* part, int nparts, double* out);
+int launch_vardiff(hipStream_t st, const sem_mesh* m, const double* nu, double* G11, double* G12,
+                   double* G22);
 
 int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* y);
 int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y);

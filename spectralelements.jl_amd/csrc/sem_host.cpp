@@ -628,6 +628,44 @@ int sem_wdot(sem_ctx* ctx, const sem_mesh* m, const double* a, const double* b, 
   return SEM_OK;
 }
 
+int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
+            double* result) {
+  SEM_ARG(ctx && a && b && result && n >= 0, "bad arguments");
+  const int np = 256;
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * (np + 1)));
+  double* part = static_cast<double*>(ctx->ws);
+  SEM_TRY(launch_dot(ctx->stream, n, a, b, w, part, np, part + np));
+  SEM_HIP(hipMemcpyAsync(result, part + np, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));
+  return SEM_OK;
+}
+
+int sem_combine(sem_ctx* ctx, const sem_mesh* m, int nterms, const double* coef,
+                const double* const* x, const int* weighted, int bc_bits, int assemble,
+                double* y) {
+  SEM_ARG(ctx && m && y, "NULL argument");
+  SEM_ARG(nterms >= 0 && nterms <= kMaxTerms, "sem_combine: 0 <= nterms <= 8");
+  SEM_ARG(nterms == 0 || (coef && x), "sem_combine: NULL coefficient or term list");
+  SEM_ARG(bc_bits < 16, "sem_combine: bc_bits must be < 16 (or negative for no mask)");
+  CombineTerms t{};
+  t.n = nterms;
+  for (int j = 0; j < nterms; ++j) {
+    SEM_ARG(x[j], "sem_combine: NULL term");
+    t.x[j] = x[j];
+    t.c[j] = coef[j];
+    t.w[j] = weighted ? weighted[j] : 0;
+  }
+  SEM_TRY(launch_combine(ctx->stream, m, t, bc_bits, y));
+  if (assemble) SEM_TRY(launch_gsmap(ctx->stream, m->map, y, y));
+  return SEM_OK;
+}
+
+int sem_vardiff_factors(sem_ctx* ctx, const sem_mesh* m, const double* nu, double* G11,
+                        double* G12, double* G22) {
+  SEM_ARG(ctx && m && nu && G11 && G12 && G22, "NULL argument");
+  return launch_vardiff(ctx->stream, m, nu, G11, G12, G22);
+}
+
 int sem_norm_inf(sem_ctx* ctx, int64_t n, const double* a, double* result) {
   SEM_ARG(ctx && a && result && n >= 0, "bad arguments");
   const int np = 256;

@@ -10,5 +10,12 @@ from .core import (  # noqa: F401
     ABu, Diagonal, GSMap, Mesh, PCGInfo, SEMOperator, adjoint_dnu, advect, ctx, deform_affine01,
     deform_sinsin, derivMat, empty_field, fixU, gatherScatter, gausslobatto, generateMask, grad,
     gradT, hlmz, identity, interpMat, lapl, mask, mass, ndgrid, norm_inf, pcg, pcg_, pcg_mixed,
-    semmesh, synchronize, to_device, to_host, wdot,
+    semmesh, synchronize, to_device, to_host, wdot, dot, combine, vardiff_factors,
 )
+from .adjoint import (  # noqa: F401
+    VarDiffusion, VarDiffusionOperator, diffusion_solver, linsolve, nodal_gradient,
+)
+from .cd import (  # noqa: F401
+    CDGradient, ConvectionDiffusion, Field, TimeStepper, bdfExtK, cd_gradient, cd_objective,
+)
+from . import workloads  # noqa: F401

@@ -89,6 +89,9 @@ SIGNATURES = {
     "sem_pcg_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _i, C.POINTER(sem_pcg_info)],
     "sem_wdot": [_vp, _vp, _vp, _vp, _dp],
     "sem_norm_inf": [_vp, _i64, _vp, _dp],
+    "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],
+    "sem_combine": [_vp, _vp, _i, _dp, _pvp, C.POINTER(C.c_int), _i, _i, _vp],
+    "sem_vardiff_factors": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sem_grad": [_vp, _vp, _vp, _vp, _vp],
     "sem_grad_t": [_vp, _vp, _vp, _vp, _vp],
     "sem_adjoint_dnu": [_vp, _vp, _vp, _vp, _vp],

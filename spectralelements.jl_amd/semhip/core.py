@@ -502,6 +502,47 @@ def wdot(a, b, msh: Mesh) -> float:
     return r.value
 
 
+def dot(a, b, w=None) -> float:
+    """sum(a .* b .* w) (w None: sum(a .* b)) over all local nodes."""
+    a, b = to_device(a), to_device(b)
+    wd = to_device(w) if w is not None else None
+    r = C.c_double()
+    L.call("sem_dot", ctx(), a.numel(), _p(a), _p(b), _p(wd), C.byref(r))
+    return r.value
+
+
+def combine(msh: Mesh, terms, M=None, assemble: bool = False, out=None):
+    """[gatherScatter](mask(sum_j c_j (B .* if weighted_j) x_j, M)) for
+    terms = [(c_j, x_j, weighted_j), ...] (at most 8): the composition of
+    makeRHS! (convectionDiffusion.jl:93-110, diffusion.jl:51-65). M must come
+    from generateMask (or be None: no mask). ``out`` may be one of the x_j."""
+    if len(terms) > 8:
+        raise ValueError("combine: at most 8 terms")
+    xs = [to_device(x) for _, x, _ in terms]
+    y = out if out is not None else empty_field(*msh.shape)
+    n = len(terms)
+    coef = (C.c_double * max(n, 1))(*[float(c) for c, _, _ in terms])
+    ptrs = (C.c_void_p * max(n, 1))(*[x.data_ptr() for x in xs])
+    wts = (C.c_int * max(n, 1))(*[int(bool(w)) for _, _, w in terms])
+    bits = -1
+    if M is not None:
+        bits = getattr(M, "_sem_bits", None)
+        if bits is None:
+            raise ValueError("combine needs a mask from generateMask")
+    L.call("sem_combine", ctx(), msh.h, n, coef, ptrs, wts, int(bits), int(bool(assemble)),
+           _p(y))
+    return y
+
+
+def vardiff_factors(nu, msh: Mesh):
+    """(nu .* G11, nu .* G12, nu .* G22): the geometric factors of the
+    variable-coefficient operator -div(nu grad u) (semPS.jl:161-163)."""
+    nu = to_device(nu)
+    g = [empty_field(*msh.shape) for _ in range(3)]
+    L.call("sem_vardiff_factors", ctx(), msh.h, _p(nu), _p(g[0]), _p(g[1]), _p(g[2]))
+    return tuple(g)
+
+
 def norm_inf(a) -> float:
     a = to_device(a)
     r = C.c_double()
@@ -690,10 +731,12 @@ def pcg_(x, b, opA, **kw):
 
 
 def pcg_mixed(b, op64: SEMOperator, op_lo: SEMOperator, d=None, tol: float = 1e-10,
-              inner_tol: float = 1e-5, max_outer: int = 200, max_inner: int = 100000,
+              inner_tol: float = 1e-2, max_outer: int = 200, max_inner: int = 1000000,
               info: Optional[PCGInfo] = None):
-    """FP64 defect correction around a low-precision fused PCG (the mixed
-    precision solve); d is the FP64 Jacobi diagonal or None."""
+    """Mixed-precision solve: a low-precision fused PCG (FP32 vectors, op_lo's
+    contractions) with FP64 residual replacement every time its residual
+    drops by inner_tol, x accumulated in FP64, until the true FP64 residual
+    norm(b - op64(x), Inf) <= tol. d is the FP64 Jacobi diagonal or None."""
     bd = to_device(b)
     x = _like(bd)
     pi = L.sem_pcg_info()
