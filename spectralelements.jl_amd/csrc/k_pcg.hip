@@ -51,31 +51,116 @@ k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
   }
 }
 
+// rows whose facts are tabulated in LDS (64 KiB); longer rows recompute them
+constexpr int kRowTable = 16384;
+
+// x-direction facts of local row i, packed: bit 0 = half multiplicity in x
+// (mult1), bits 1-2 = seam kind of the strip kernel's unsummed copies
+// (0 none, 1 pair (i-1, i), 2 pair (i, i+1), 3 periodic wrap (L-1, 0)),
+// bit 3 = masked by an x Dirichlet side.
+__device__ inline int row_info(int i, const SeamGeom& g) {
+  int v = 0;
+  const int a = i % g.P;
+  if ((a == 0 && (i > 0 || g.per_x)) || (a == g.P - 1 && (i < g.L - 1 || g.per_x))) v |= 1;
+  const int r = i % g.CXP;
+  int kind = 0;
+  if (r == 0 && i > 0) kind = 1;
+  else if (r == g.CXP - 1 && i < g.L - 1) kind = 2;
+  else if (g.per_x && (i == 0 || i == g.L - 1)) kind = 3;
+  v |= kind << 1;
+  if (((g.bc & SEM_BC_XMIN) && i == 0) || ((g.bc & SEM_BC_XMAX) && i == g.L - 1)) v |= 8;
+  return v;
+}
+
+// x += alpha p; r -= alpha w (w's seam copies finished here with the k_seams
+// rules: same operands, same order, so every copy gets identical bits);
+// partials of <r, z>_mult and norm(r, Inf) for the next iteration. A block
+// walks whole columns j (its share of NC); each thread takes rows tid,
+// tid + kUpdThreads, ...; the per-row facts come from an LDS table built once per
+// block and the per-column facts are uniform, so no per-node integer division.
+constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP32
+
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kUpdThreads)
 k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
              const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
              int nQ, PcgState* __restrict__ st, int k, double* __restrict__ part, int n3,
              SeamGeom sg, long n) {
-  const int P = sg.P, L = sg.L, NC = sg.NC, per_x = sg.per_x, per_y = sg.per_y;
-  __shared__ double red[256];
+  const int P = sg.P, NC = sg.NC, per_y = sg.per_y;
+  const int Lr = sg.L;
+  const long L = sg.L;
+  __shared__ double red[kUpdThreads];
+  extern __shared__ int rinfo[];
   if (block_done(st, red)) return;
-  const double pw = reduce_sum_array(Q, nQ, red, 256);
+  const bool table = Lr <= kRowTable;   // uniform
+  if (table)
+    for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
+  const double pw = reduce_sum_array(Q, nQ, red, kUpdThreads);   // synchronises the block
   const T alpha = T(st->t[k & 1] / pw);
+  const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int j0 = blockIdx.x * cpb;
+  const int j1 = min(NC, j0 + cpb);
   double rho = 0.0, mx = 0.0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % L);
-    const int j = (int)(t / L);
-    x[t] = x[t] + alpha * p[t];
-    const T rv = r[t] - alpha * seam_final(w, i, j, sg);
-    r[t] = rv;
-    const T z = d ? d[t] * rv : rv;
-    rho += (double)rv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
-    mx = nanmax(mx, fabs((double)rv));
+  for (int j = j0; j < j1; ++j) {
+    const int b = j % P;
+    const bool yhalf = (b == 0 && (j > 0 || per_y)) || (b == P - 1 && (j < NC - 1 || per_y));
+    const double my = yhalf ? 0.5 : 1.0;
+    int jl = -1, ju = -1;   // element y-interface (for x-seam copies)
+    if (b == 0 && (j > 0 || per_y)) {
+      jl = j > 0 ? j - 1 : NC - 1;
+      ju = j;
+    } else if (b == P - 1 && (j < NC - 1 || per_y)) {
+      jl = j;
+      ju = j + 1 < NC ? j + 1 : 0;
+    }
+    const int qy = j % sg.SP;   // segment y-seam (for the other copies)
+    int jL = -1, jR = -1;
+    if (qy == 0 && j > 0) {
+      jL = j - 1;
+      jR = j;
+    } else if (qy == sg.SP - 1 && j < NC - 1) {
+      jL = j;
+      jR = j + 1;
+    } else if (per_y && (j == 0 || j == NC - 1)) {
+      jL = NC - 1;
+      jR = 0;
+    }
+    const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
+    const long cj = L * j;
+#pragma unroll 2
+    for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
+      const int info = table ? rinfo[i] : row_info(i, sg);
+      const long t = i + cj;
+      const int kind = (info >> 1) & 3;
+      const T mv = ((info & 8) || masky) ? T(0) : T(1);
+      T wf;
+      if (kind) {
+        const int iL = kind == 1 ? i - 1 : kind == 2 ? i : Lr - 1;
+        const int iR = kind == 1 ? i : kind == 2 ? i + 1 : 0;
+        T sv;
+        if (jl >= 0) {
+          const T sl = w[iL + L * jl] + w[iR + L * jl];
+          const T su = w[iL + L * ju] + w[iR + L * ju];
+          sv = sl + su;
+        } else {
+          sv = w[iL + cj] + w[iR + cj];
+        }
+        wf = sv * mv;
+      } else if (jL >= 0) {
+        wf = (w[i + L * jL] + w[i + L * jR]) * mv;
+      } else {
+        wf = w[t];
+      }
+      x[t] = x[t] + alpha * p[t];
+      const T rv = r[t] - alpha * wf;
+      r[t] = rv;
+      const T z = d ? d[t] * rv : rv;
+      rho += (double)rv * (double)z * ((info & 1 ? 0.5 : 1.0) * my);
+      mx = nanmax(mx, fabs((double)rv));
+    }
   }
-  rho = block_sum(rho, red, 256);
-  mx = block_max(mx, red, 256);
+  rho = block_sum(rho, red, kUpdThreads);
+  mx = block_max(mx, red, kUpdThreads);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = mx;
@@ -254,8 +339,9 @@ struct PcgRun {
         pc.dev.maxiter = maxiter;
         pc.dev.k = k;
         SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
-        hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(256), 0, st, x, r, p, w, d, Q, nQ,
-                           state, k, part + 2 * n3 * (k & 1), n3, sg, n);
+        hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(kUpdThreads),
+                           g.L <= kRowTable ? sizeof(int) * g.L : 0, st, x, r,
+                           p, w, d, Q, nQ, state, k, part + 2 * n3 * (k & 1), n3, sg, n);
       }
       SEM_HIP(hipGetLastError());
       SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
