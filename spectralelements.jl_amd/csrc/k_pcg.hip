@@ -276,26 +276,9 @@ struct PcgRun {
     state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb);
     SEM_TRY(ensure_pinned(ctx, 128));  // PcgState, then a scalar slot at byte 64
     hs = static_cast<PcgState*>(ctx->pinned);
-    aa.u = nullptr;
-    aa.y = w;
-    aa.G11 = op->G11;
-    aa.G12 = op->G12;
-    aa.G22 = op->G22;
-    aa.B = op->B;
-    aa.Dr = sizeof(T) == 8 ? (const void*)m->Dr : (const void*)m->Dr32;
-    aa.Ds = sizeof(T) == 8 ? (const void*)m->Ds : (const void*)m->Ds32;
-    aa.nu_f = op->nu_f;
-    aa.k_f = op->k_f;
-    aa.nu = op->desc.nu;
-    aa.k = op->desc.k;
-    aa.has_mass = op->has_mass ? 1 : 0;
+    aa = args_of(op, nullptr, w);   // factors, precision, recomputed geometry
     aa.bc = op->desc.bc_bits;
     aa.assemble = 1;
-    aa.mfma = op->precision == SEM_F16_MFMA     ? 1
-              : op->precision == SEM_BF16_MFMA  ? 2
-              : op->precision == SEM_F16X3_MFMA ? 3
-              : op->precision == SEM_BF16X3_MFMA ? 4
-                                                 : 0;
     return SEM_OK;
   }
 

@@ -131,6 +131,8 @@ struct ApplyArgs {
 };
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);
+// launch arguments of an operator apply y = op(u)
+ApplyArgs args_of(const sem_operator* op, const void* u, void* y);
 
 // Device-resident PCG scalars (pcg.jl:23-55). t[k & 1] holds t = <r, h> of
 // iteration k; done/iters/converged/res are set once, by the first kernel of

@@ -149,6 +149,8 @@ int mfma_mode(int precision) {
   }
 }
 
+}  // namespace
+
 ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   const sem_mesh* m = op->mesh;
   ApplyArgs a{};
@@ -171,6 +173,8 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   a.mfma = mfma_mode(op->precision);
   return a;
 }
+
+namespace {
 
 void free_mesh(sem_mesh* m) {
   if (!m) return;
