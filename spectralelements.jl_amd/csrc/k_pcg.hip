@@ -72,12 +72,31 @@ __device__ inline int row_info(int i, const SeamGeom& g) {
   return v;
 }
 
+// One node of the PCG update: x += alpha p, r -= alpha wf, partials of
+// <r, z>_mult (m = the node's mult) and norm(r, Inf).
+template <typename T>
+__device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
+                                         const T* __restrict__ p, const T* __restrict__ d,
+                                         long t, T wf, T alpha, double m, double& rho,
+                                         double& mx) {
+  x[t] = x[t] + alpha * p[t];
+  const T rv = r[t] - alpha * wf;
+  r[t] = rv;
+  const T z = d ? d[t] * rv : rv;
+  rho += (double)rv * (double)z * m;
+  mx = nanmax(mx, fabs((double)rv));
+}
+
 // x += alpha p; r -= alpha w (w's seam copies finished here with the k_seams
 // rules: same operands, same order, so every copy gets identical bits);
 // partials of <r, z>_mult and norm(r, Inf) for the next iteration. A block
-// walks whole columns j (its share of NC); each thread takes rows tid,
-// tid + kUpdThreads, ...; the per-row facts come from an LDS table built once per
-// block and the per-column facts are uniform, so no per-node integer division.
+// walks whole columns j (its share of NC). Bulk pass: each thread takes rows
+// tid, tid + kUpdThreads, ... except the strip-seam rows; the per-row facts
+// come from an LDS table built once per block, the per-column facts (segment
+// y-seam, mask, mult) are uniform, so the bulk has no per-node integer
+// division and no divergent seam branches. Seam pass: the 2 (nstrips - 1)
+// strip-seam rows (+ 2 periodic wrap rows) of the block's columns, with the
+// four-copy sums at element y-interfaces.
 constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP32
 
 template <typename T>
@@ -95,25 +114,20 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   const bool table = Lr <= kRowTable;   // uniform
   if (table)
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
-  const double pw = reduce_sum_array(Q, nQ, red, kUpdThreads);   // synchronises the block
+  double qs = 0.0;
+  for (int q = threadIdx.x; q < nQ; q += kUpdThreads) qs += Q[q];
+  const double pw = block_sum_w<kUpdThreads>(qs, red);   // synchronises the block
   const T alpha = T(st->t[k & 1] / pw);
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
   const int j0 = blockIdx.x * cpb;
   const int j1 = min(NC, j0 + cpb);
   double rho = 0.0, mx = 0.0;
+  // bulk: every row but the strip-seam rows
   for (int j = j0; j < j1; ++j) {
     const int b = j % P;
     const bool yhalf = (b == 0 && (j > 0 || per_y)) || (b == P - 1 && (j < NC - 1 || per_y));
     const double my = yhalf ? 0.5 : 1.0;
-    int jl = -1, ju = -1;   // element y-interface (for x-seam copies)
-    if (b == 0 && (j > 0 || per_y)) {
-      jl = j > 0 ? j - 1 : NC - 1;
-      ju = j;
-    } else if (b == P - 1 && (j < NC - 1 || per_y)) {
-      jl = j;
-      ju = j + 1 < NC ? j + 1 : 0;
-    }
-    const int qy = j % sg.SP;   // segment y-seam (for the other copies)
+    const int qy = j % sg.SP;   // segment y-seam
     int jL = -1, jR = -1;
     if (qy == 0 && j > 0) {
       jL = j - 1;
@@ -127,40 +141,71 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     }
     const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
     const long cj = L * j;
+    if (jL >= 0) {   // uniform: a segment y-seam column
+      const long cL = L * jL, cR = L * jR;
 #pragma unroll 2
-    for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
-      const int info = table ? rinfo[i] : row_info(i, sg);
-      const long t = i + cj;
-      const int kind = (info >> 1) & 3;
-      const T mv = ((info & 8) || masky) ? T(0) : T(1);
-      T wf;
-      if (kind) {
-        const int iL = kind == 1 ? i - 1 : kind == 2 ? i : Lr - 1;
-        const int iR = kind == 1 ? i : kind == 2 ? i + 1 : 0;
-        T sv;
-        if (jl >= 0) {
-          const T sl = w[iL + L * jl] + w[iR + L * jl];
-          const T su = w[iL + L * ju] + w[iR + L * ju];
-          sv = sl + su;
-        } else {
-          sv = w[iL + cj] + w[iR + cj];
-        }
-        wf = sv * mv;
-      } else if (jL >= 0) {
-        wf = (w[i + L * jL] + w[i + L * jR]) * mv;
-      } else {
-        wf = w[t];
+      for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
+        const int info = table ? rinfo[i] : row_info(i, sg);
+        if (info & 6) continue;
+        const T mv = ((info & 8) || masky) ? T(0) : T(1);
+        const T wf = (w[i + cL] + w[i + cR]) * mv;
+        upd_node(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my, rho, mx);
       }
-      x[t] = x[t] + alpha * p[t];
-      const T rv = r[t] - alpha * wf;
-      r[t] = rv;
-      const T z = d ? d[t] * rv : rv;
-      rho += (double)rv * (double)z * ((info & 1 ? 0.5 : 1.0) * my);
-      mx = nanmax(mx, fabs((double)rv));
+    } else {
+#pragma unroll 2
+      for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
+        const int info = table ? rinfo[i] : row_info(i, sg);
+        if (info & 6) continue;
+        const long t = i + cj;
+        upd_node(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my, rho, mx);
+      }
     }
   }
-  rho = block_sum(rho, red, kUpdThreads);
-  mx = block_max(mx, red, kUpdThreads);
+  // strip-seam rows: s*CXP - 1 and s*CXP for s = 1 .. nstrips-1, then the
+  // periodic wrap rows 0 and L-1
+  const int nsr = 2 * (sg.nstrips - 1);
+  const int nxr = nsr + (sg.per_x ? 2 : 0);
+  const int ncol = max(0, j1 - j0);
+  for (int item = threadIdx.x; item < nxr * ncol; item += kUpdThreads) {
+    const int q = item % nxr;
+    const int j = j0 + item / nxr;
+    int i, iL, iR;
+    if (q < nsr) {
+      const int s = q / 2 + 1;
+      i = (q & 1) ? s * sg.CXP : s * sg.CXP - 1;
+      iL = s * sg.CXP - 1;
+      iR = s * sg.CXP;
+    } else {
+      i = (q == nsr) ? 0 : Lr - 1;
+      iL = Lr - 1;
+      iR = 0;
+    }
+    const int info = row_info(i, sg);
+    const int b = j % P;
+    const bool yhalf = (b == 0 && (j > 0 || per_y)) || (b == P - 1 && (j < NC - 1 || per_y));
+    int jl = -1, ju = -1;   // element y-interface: four copies
+    if (b == 0 && (j > 0 || per_y)) {
+      jl = j > 0 ? j - 1 : NC - 1;
+      ju = j;
+    } else if (b == P - 1 && (j < NC - 1 || per_y)) {
+      jl = j;
+      ju = j + 1 < NC ? j + 1 : 0;
+    }
+    const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
+    const T mv = ((info & 8) || masky) ? T(0) : T(1);
+    T sv;
+    if (jl >= 0) {
+      const T sl = w[iL + L * jl] + w[iR + L * jl];
+      const T su = w[iL + L * ju] + w[iR + L * ju];
+      sv = sl + su;
+    } else {
+      sv = w[iL + L * j] + w[iR + L * j];
+    }
+    upd_node(x, r, p, d, i + L * j, sv * mv, alpha,
+             (info & 1 ? 0.5 : 1.0) * (yhalf ? 0.5 : 1.0), rho, mx);
+  }
+  rho = block_sum_w<kUpdThreads>(rho, red);
+  mx = block_max_w<kUpdThreads>(mx, red);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = mx;

@@ -9,9 +9,23 @@ __device__ __forceinline__ double nanmax(double a, double b) {
   return (b > a || b != b) ? b : a;
 }
 
+// Fixed-order reductions within a 64-lane wavefront: a butterfly over the
+// lanes; lane 0's result has a fixed operand order and is the one used.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 // Fixed-order block sum of v over nthreads threads (red has nthreads entries,
 // nthreads need not be a power of two). Every block computes bit-identical
-// results for identical inputs.
+// results for identical inputs. (A tree over LDS: it adds no registers to
+// the register-heavy apply kernels that call it.)
 __device__ inline double block_sum(double v, double* red, int nthreads) {
   const int tid = threadIdx.x;
   red[tid] = v;
@@ -39,6 +53,45 @@ __device__ inline double block_max(double v, double* red, int nthreads) {
     __syncthreads();
     n = h;
   }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// The same reductions for blocks of NT threads, NT a multiple of 64: lane
+// shuffles within each wavefront, then one LDS step across the waves (three
+// barriers instead of log2(NT)). Fixed order, bit-identical across blocks.
+template <int NT>
+__device__ inline double block_sum_w(double v, double* red) {
+  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  v = wave_sum(v);
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    double t = lane < NT / 64 ? red[lane] : 0.0;
+    t = wave_sum(t);
+    if (lane == 0) red[0] = t;   // wave 0 has read red[0..NT/64) already
+  }
+  __syncthreads();
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <int NT>
+__device__ inline double block_max_w(double v, double* red) {
+  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  v = wave_max(v);
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    double t = lane < NT / 64 ? red[lane] : 0.0;
+    t = wave_max(t);
+    if (lane == 0) red[0] = t;
+  }
+  __syncthreads();
   const double r = red[0];
   __syncthreads();
   return r;
