@@ -73,18 +73,25 @@ __device__ inline int row_info(int i, const SeamGeom& g) {
 }
 
 // One node of the PCG update: x += alpha p, r -= alpha wf, partials of
-// <r, z>_mult (m = the node's mult) and norm(r, Inf).
-template <typename T>
+// <r, z>_mult (m = the node's mult) and norm(r, Inf). With keep == false the
+// node is left as it is (its old x and r are stored back) and adds nothing to
+// the partials: a select, not a branch, so the bulk pass has no divergent
+// control flow. The max ignores NaN here; the block restores NaN from the
+// <r, z> partial, which is NaN whenever some r is.
+template <typename T, bool HAS_D>
 __device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
                                          const T* __restrict__ p, const T* __restrict__ d,
-                                         long t, T wf, T alpha, double m, double& rho,
-                                         double& mx) {
-  x[t] = x[t] + alpha * p[t];
-  const T rv = r[t] - alpha * wf;
-  r[t] = rv;
-  const T z = d ? d[t] * rv : rv;
-  rho += (double)rv * (double)z * m;
-  mx = nanmax(mx, fabs((double)rv));
+                                         long t, T wf, T alpha, double m, bool keep,
+                                         double& rho, double& mx) {
+  const T xo = x[t], ro = r[t];
+  const T xn = xo + alpha * p[t];
+  const T rv = ro - alpha * wf;
+  x[t] = keep ? xn : xo;
+  r[t] = keep ? rv : ro;
+  T z = rv;
+  if constexpr (HAS_D) z = d[t] * rv;
+  rho += (double)rv * (double)z * (keep ? m : 0.0);
+  mx = fmax(mx, keep ? fabs((double)rv) : 0.0);
 }
 
 // x += alpha p; r -= alpha w (w's seam copies finished here with the k_seams
@@ -99,7 +106,10 @@ __device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
 // four-copy sums at element y-interfaces.
 constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP32
 
-template <typename T>
+// HAS_D: a diagonal preconditioner is present. FULL: L is a multiple of
+// kUpdThreads and the row table fits, so the bulk row loop has a uniform trip
+// count (scalar loop control, no per-lane exit masks).
+template <typename T, bool HAS_D, bool FULL>
 __global__ void __launch_bounds__(kUpdThreads)
 k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
              const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
@@ -111,7 +121,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   __shared__ double red[kUpdThreads];
   extern __shared__ int rinfo[];
   if (block_done(st, red)) return;
-  const bool table = Lr <= kRowTable;   // uniform
+  const bool table = FULL || Lr <= kRowTable;   // uniform
   if (table)
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
   double qs = 0.0;
@@ -141,26 +151,47 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     }
     const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
     const long cj = L * j;
-    if (jL >= 0) {   // uniform: a segment y-seam column
-      const long cL = L * jL, cR = L * jR;
+    if constexpr (FULL) {
+      if (jL >= 0) {   // uniform: a segment y-seam column
+        const long cL = L * jL, cR = L * jR;
 #pragma unroll 2
+        for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
+          const int i = i0 + (int)threadIdx.x;
+          const int info = rinfo[i];
+          const T mv = ((info & 8) || masky) ? T(0) : T(1);
+          const T wf = (w[i + cL] + w[i + cR]) * mv;
+          upd_node<T, HAS_D>(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
+                             !(info & 6), rho, mx);
+        }
+      } else {
+#pragma unroll 2
+        for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
+          const int i = i0 + (int)threadIdx.x;
+          const int info = rinfo[i];
+          const long t = i + cj;
+          upd_node<T, HAS_D>(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
+                             !(info & 6), rho, mx);
+        }
+      }
+    } else if (jL >= 0) {
+      const long cL = L * jL, cR = L * jR;
       for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
         const int info = table ? rinfo[i] : row_info(i, sg);
-        if (info & 6) continue;
         const T mv = ((info & 8) || masky) ? T(0) : T(1);
         const T wf = (w[i + cL] + w[i + cR]) * mv;
-        upd_node(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my, rho, mx);
+        upd_node<T, HAS_D>(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
+                           !(info & 6), rho, mx);
       }
     } else {
-#pragma unroll 2
       for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
         const int info = table ? rinfo[i] : row_info(i, sg);
-        if (info & 6) continue;
         const long t = i + cj;
-        upd_node(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my, rho, mx);
+        upd_node<T, HAS_D>(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
+                           !(info & 6), rho, mx);
       }
     }
   }
+  __syncthreads();   // the bulk stored the seam rows' old x, r; the seam pass follows
   // strip-seam rows: s*CXP - 1 and s*CXP for s = 1 .. nstrips-1, then the
   // periodic wrap rows 0 and L-1
   const int nsr = 2 * (sg.nstrips - 1);
@@ -201,14 +232,14 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     } else {
       sv = w[iL + L * j] + w[iR + L * j];
     }
-    upd_node(x, r, p, d, i + L * j, sv * mv, alpha,
-             (info & 1 ? 0.5 : 1.0) * (yhalf ? 0.5 : 1.0), rho, mx);
+    upd_node<T, HAS_D>(x, r, p, d, i + L * j, sv * mv, alpha,
+                       (info & 1 ? 0.5 : 1.0) * (yhalf ? 0.5 : 1.0), true, rho, mx);
   }
   rho = block_sum_w<kUpdThreads>(rho, red);
   mx = block_max_w<kUpdThreads>(mx, red);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
-    part[n3 + blockIdx.x] = mx;
+    part[n3 + blockIdx.x] = rho != rho ? rho : mx;   // a NaN residual makes rho NaN
   }
 }
 
@@ -347,6 +378,23 @@ struct PcgRun {
     return SEM_OK;
   }
 
+  void launch_update(hipStream_t st, T* x, int k) {
+    const size_t lds = g.L <= kRowTable ? sizeof(int) * g.L : 0;
+    const bool full = g.L % kUpdThreads == 0 && g.L <= kRowTable;
+    double* pk = part + 2 * n3 * (k & 1);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, x, r, p, w, d, Q, nQ, state,
+                         k, pk, n3, sg, n);
+    };
+    if (d) {
+      if (full) go(k_pcg_update<T, true, true>);
+      else go(k_pcg_update<T, true, false>);
+    } else {
+      if (full) go(k_pcg_update<T, false, true>);
+      else go(k_pcg_update<T, false, false>);
+    }
+  }
+
   // launches iterations k_start.. until the stop test fires (norm(r, Inf) <=
   // tol or k - 1 == maxiter); on return hs holds the final state
   int run(T* x, double tol, int maxiter, int k_start) {
@@ -367,9 +415,7 @@ struct PcgRun {
         pc.dev.maxiter = maxiter;
         pc.dev.k = k;
         SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
-        hipLaunchKernelGGL((k_pcg_update<T>), dim3(n3), dim3(kUpdThreads),
-                           g.L <= kRowTable ? sizeof(int) * g.L : 0, st, x, r,
-                           p, w, d, Q, nQ, state, k, part + 2 * n3 * (k & 1), n3, sg, n);
+        launch_update(st, x, k);
       }
       SEM_HIP(hipGetLastError());
       SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
