@@ -216,7 +216,7 @@ int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
  * vectors; its iterate is accumulated into x in FP64 and its recursive
  * residual is replaced by the true residual b - op64(x) (FP64 operator)
  * whenever it has dropped by the factor inner_tol since the last replacement
- * or after 1000 inner iterations. The search direction and iteration count
+ * (for the FP32 operator also after 1000 inner iterations). The search direction and iteration count
  * carry over (the Krylov space is not restarted); at each replacement the
  * inner vectors are rescaled by a power of two so the residual is O(1). A
  * cycle whose true residual is not finite or grew tenfold is rolled back and

@@ -493,7 +493,7 @@ double pow2_near(double v) { return std::ldexp(1.0, std::ilogb(v)); }
 // FP32 fused PCG whose iterate is accumulated into x in FP64 and whose
 // recursive residual is replaced by the true residual b - A64 x (fused FP64
 // operator) whenever it has dropped by the factor delta since the last
-// replacement, or after kChunk inner iterations. The search direction and the
+// replacement, or (FP32 operator) after kChunk inner iterations. The search direction and the
 // iteration count carry over, so the Krylov space is not restarted. At every
 // replacement the inner vectors are rescaled by a power of two so that the
 // residual is O(1) (FP16 contraction inputs stay in their exponent range).
@@ -583,7 +583,11 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
       if (s != SEM_OK) break;
       const double tol_c = std::max(delta * res, 0.5 * tol) / scale;
       const int k0 = k;
-      s = R.run(x32, tol_c, k0 - 1 + std::min(kChunk, std::max(1, max_inner - total)), k0);
+      // reliable updates replace the residual at least every kChunk iterations;
+      // a restarted cycle runs its inner solve to tol_c (a cap would restart
+      // the Krylov space mid-solve)
+      const int cap = reliable ? kChunk : max_inner;
+      s = R.run(x32, tol_c, k0 - 1 + std::min(cap, std::max(1, max_inner - total)), k0);
       if (s != SEM_OK) break;
       k = R.hs->iters + 1;
       total += k - k0;
