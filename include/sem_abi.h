@@ -216,12 +216,15 @@ int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
  * vectors; its iterate is accumulated into x in FP64 and its recursive
  * residual is replaced by the true residual b - op64(x) (FP64 operator)
  * whenever it has dropped by the factor inner_tol since the last replacement
- * (for the FP32 operator also after 1000 inner iterations). The search direction and iteration count
+ * (when the search direction is kept, also after 1000 inner iterations). The search direction and iteration count
  * carry over (the Krylov space is not restarted); at each replacement the
  * inner vectors are rescaled by a power of two so the residual is O(1). A
  * cycle whose true residual is not finite or grew tenfold is rolled back and
- * the Krylov space restarted; FP16/BF16-input operators restart it at every
- * replacement (classical defect correction). Stops when norm(b - op64(x), Inf) <= tol;
+ * the Krylov space restarted. FP32 and split (x3) operators keep the search
+ * direction across replacements until one shows no progress, then restart it
+ * at every replacement; single-pass FP16/BF16-input operators always restart
+ * it (classical defect correction).
+ * SEM_MIXED_RELIABLE=1 / 0 forces one policy. Stops when norm(b - op64(x), Inf) <= tol;
  * SEM_WARN_MAXITER after max_outer replacements, four replacements without a
  * 10% improvement, or max_inner total inner iterations. d64: FP64 Jacobi
  * diagonal (or NULL for identity). info->outer = replacements, info->iters =

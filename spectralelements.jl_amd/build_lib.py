@@ -73,10 +73,15 @@ def _newest_input() -> float:
     return max(t, os.path.getmtime(os.path.join(os.path.dirname(HERE), "include", "sem_abi.h")))
 
 
-def build(jobs: int = 8, verbose: bool = False, force: bool = True) -> str:
-    """Compiles every unit (force=True) or only units older than any source."""
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(jobs: int = 8, verbose: bool = False, force: bool = True, defines=(),
+          lib: str = None, objdir: str = None) -> str:
+    """Compiles every unit (force=True) or only units older than any source.
+    defines (e.g. ["-DSEM_AX_THREADS=128"]) are added to every hipcc line;
+    lib / objdir redirect the library and the objects (tuning builds)."""
+    out_lib = lib or LIB
+    bdir = objdir or BUILD
+    os.makedirs(bdir, exist_ok=True)
+    os.makedirs(os.path.dirname(out_lib), exist_ok=True)
     gen_tables(verbose)
     cc = hipcc()
     newest = _newest_input()
@@ -84,7 +89,7 @@ def build(jobs: int = 8, verbose: bool = False, force: bool = True) -> str:
     t0 = time.time()
     pending = []
     for s, extra, name in UNITS:
-        o = os.path.join(BUILD, name + ".o")
+        o = os.path.join(bdir, name + ".o")
         objs.append(o)
         if force or not os.path.exists(o) or os.path.getmtime(o) < newest:
             pending.append((s, extra, name))
@@ -92,8 +97,8 @@ def build(jobs: int = 8, verbose: bool = False, force: bool = True) -> str:
     while pending or running:
         while pending and len(running) < jobs:
             s, extra, name = pending.pop(0)
-            o = os.path.join(BUILD, name + ".o")
-            cmd = [cc, *FLAGS, *extra, "-x", "hip", "-c", os.path.join(CSRC, s), "-o", o]
+            o = os.path.join(bdir, name + ".o")
+            cmd = [cc, *FLAGS, *defines, *extra, "-x", "hip", "-c", os.path.join(CSRC, s), "-o", o]
             if verbose:
                 print(" ".join(cmd), flush=True)
             running.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -111,15 +116,15 @@ def build(jobs: int = 8, verbose: bool = False, force: bool = True) -> str:
                 print(out)
         running = still
         time.sleep(0.05)
-    tmp = LIB + ".tmp"
+    tmp = out_lib + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out_lib)
     if verbose:
-        print(f"built {LIB} in {time.time() - t0:.1f}s")
-    return LIB
+        print(f"built {out_lib} in {time.time() - t0:.1f}s")
+    return out_lib
 
 
 if __name__ == "__main__":

@@ -493,15 +493,16 @@ double pow2_near(double v) { return std::ldexp(1.0, std::ilogb(v)); }
 // FP32 fused PCG whose iterate is accumulated into x in FP64 and whose
 // recursive residual is replaced by the true residual b - A64 x (fused FP64
 // operator) whenever it has dropped by the factor delta since the last
-// replacement, or (FP32 operator) after kChunk inner iterations. The search direction and the
+// replacement, or (when the direction is kept) after kChunk inner iterations. The search direction and the
 // iteration count carry over, so the Krylov space is not restarted. At every
 // replacement the inner vectors are rescaled by a power of two so that the
 // residual is O(1) (FP16 contraction inputs stay in their exponent range).
 // A cycle whose true residual is not finite or grew tenfold is rolled back
 // and the Krylov space restarted from the last accepted x. Operators with
-// FP16/BF16 contraction inputs restart the Krylov space at every replacement
-// (classical defect correction): their operator error is too large for the
-// carried search direction to stay useful. Stops when the
+// single-pass FP16/BF16 contraction inputs restart the Krylov space at every
+// replacement (classical defect correction): their operator error is too
+// large for the carried search direction to stay useful. The others keep it
+// until a replacement shows no progress, then restart from there on. Stops when the
 // true FP64 residual meets tol; gives up after max_cycles replacements, four
 // without a 10% improvement, or max_inner inner iterations in total.
 constexpr int kChunk = 1000;
@@ -540,7 +541,13 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
     return s;
   }
   const bool debug = std::getenv("SEM_DEBUG_MIXED") != nullptr;
-  const bool reliable = op_lo->precision == SEM_F32;
+  // The FP32 and split (x3) operators keep the search direction across
+  // replacements; single-pass FP16/BF16 operators restart it.
+  // SEM_MIXED_RELIABLE=1 / 0 forces one policy.
+  const char* menv = std::getenv("SEM_MIXED_RELIABLE");
+  bool reliable = menv ? std::atoi(menv) != 0
+                       : (op_lo->precision == SEM_F32 || op_lo->precision == SEM_F16X3_MFMA ||
+                          op_lo->precision == SEM_BF16X3_MFMA);
   hipEvent_t e0, e1;
   SEM_HIP(hipEventCreate(&e0));
   SEM_HIP(hipEventCreate(&e1));
@@ -613,6 +620,9 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
       }
       stall = (res < 0.9 * best) ? 0 : stall + 1;
       best = std::min(best, res);
+      // a replacement without progress: the kept direction has gone stale,
+      // restart the Krylov space from here on
+      if (stall > 0 && reliable && !menv) reliable = false;
       if (cycles >= max_cycles || stall >= 4 || total >= max_inner) break;
       if (bad || !reliable) {
         s = start_inner();

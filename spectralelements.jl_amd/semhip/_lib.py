@@ -103,9 +103,12 @@ _lib = None
 _lock = threading.Lock()
 
 
-def load(path: str = LIB_PATH):
-    """Loads libsemhip.so and declares every entry point; raises if missing."""
+def load(path: str = None):
+    """Loads libsemhip.so (the in-tree build, or the file named by the
+    environment variable SEMHIP_LIB) and declares every entry point; raises
+    if it is missing."""
     global _lib
+    path = path or os.environ.get("SEMHIP_LIB") or LIB_PATH
     with _lock:
         if _lib is not None:
             return _lib
