@@ -163,6 +163,16 @@ int sem_lapl(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);
 int sem_hlmz(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double nu,
              const double* nu_f, double k, const double* k_f, double* y);
 
+/* wr = G11 .* a + G12 .* b, ws = G12 .* a + G22 .* b over n entries: the
+ * pointwise step of laplace (lapl.jl:75-76) and of its dealiased form on the
+ * interpolation grid (lapl.jl:95-96) */
+int sem_gprod(sem_ctx* ctx, int64_t n, const double* G11, const double* G12, const double* G22,
+              const double* a, const double* b, double* wr, double* ws);
+/* out = alpha .* x .+ beta .* y over n entries (the broadcast sums of
+ * lapl.jl:78 and hlmz.jl:15-16); out may alias x or y */
+int sem_axpby(sem_ctx* ctx, int64_t n, double alpha, const double* x, double beta,
+              const double* y, double* out);
+
 /* ------------------------------------------------------- fused operator */
 typedef struct sem_op_desc {
   double nu;             /* scalar nu (used when nu_f == NULL) */

@@ -36,6 +36,7 @@ __all__ = [
     "grad", "gradT", "advect", "bdfExtK", "TimeStepper", "jacobi_diag",
     "deform_sinsin", "deform_affine01", "fixU", "local_to_global",
     "advect_transpose", "vardiff_operator", "vardiff_gradient", "cd_forward",
+    "laplace_dealiased", "lapl12", "mass8",
     "cd_adjoint", "cd_objective",
 ]
 
@@ -392,14 +393,41 @@ def lapl(u, msh: Mesh, nu=None):
     return Au
 
 
+def laplace_dealiased(u, Jr, Js, Dr, Ds, G11, G12, G22):
+    """8-argument laplace (lapl.jl:83-103): derivatives on the GLL grid,
+    interpolated by (Js kron Jr), G applied on the interpolation grid,
+    projected back by the transposes, then Dr' / Ds'."""
+    ur = ABu(None, Dr, u)
+    us = ABu(Ds, None, u)
+    Jur = ABu(Js, Jr, ur)
+    Jus = ABu(Js, Jr, us)
+    vr = G11 * Jur + G12 * Jus
+    vs = G12 * Jur + G22 * Jus
+    wr = ABu(Js.T, Jr.T, vr)
+    ws = ABu(Js.T, Jr.T, vs)
+    return ABu(None, Dr.T, wr) + ABu(Ds.T, None, ws)
+
+
 def lapl12(u, M, Jr, Js, QQtx, QQty, Dr, Ds, G11, G12, G22, mult):
-    """12-argument lapl with gather-scatter and mask (lapl.jl:54-68), for the
-    undealiased case Jr = Js = [] (the Zygote hook is the identity forward)."""
-    if not (_empty(Jr) and _empty(Js)):
-        raise NotImplementedError("dealiased lapl is outside the oracle")
-    Au = laplace(u, Dr, Ds, G11, G12, G22)
+    """12-argument lapl with gather-scatter and mask (lapl.jl:54-68); Jr = Js
+    = [] is the plain laplace (ABu of [] is the identity); the Zygote hook is
+    the identity in the forward pass."""
+    if _empty(Jr) and _empty(Js):
+        Au = laplace(u, Dr, Ds, G11, G12, G22)
+    else:
+        Au = laplace_dealiased(u, Jr, Js, Dr, Ds, G11, G12, G22)
     Au = gatherScatter(Au, QQtx, QQty)
     return mask(Au, M)
+
+
+def mass8(u, M, B, Jr, Js, QQtx, QQty, mult):
+    """8-argument mass (mass.jl:32-50): Js'Jr'(B .* (Js kron Jr) u) (B empty:
+    no weight), then gather-scatter and mask."""
+    Ju = ABu(Js, Jr, u)
+    BJu = Ju if _empty(B) else B * Ju
+    Bu = ABu(None if _empty(Js) else Js.T, None if _empty(Jr) else Jr.T, BJu)
+    Bu = gatherScatter(Bu, QQtx, QQty)
+    return mask(Bu, M)
 
 
 def mass(u, msh: Mesh):

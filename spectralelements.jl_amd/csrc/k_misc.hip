@@ -302,6 +302,25 @@ k_dot_part(const double* __restrict__ a, const double* __restrict__ b,
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// wr = G11 a + G12 b, ws = G12 a + G22 b (lapl.jl:75-76, 95-96)
+__global__ void k_gprod(const double* __restrict__ G11, const double* __restrict__ G12,
+                        const double* __restrict__ G22, const double* __restrict__ a,
+                        const double* __restrict__ b, double* __restrict__ wr,
+                        double* __restrict__ ws, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double av = a[t], bv = b[t];
+  wr[t] = G11[t] * av + G12[t] * bv;
+  ws[t] = G12[t] * av + G22[t] * bv;
+}
+
+// out = alpha x + beta y (out may alias x or y)
+__global__ void k_axpby(double alpha, const double* x, double beta, const double* y, double* out,
+                        long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = alpha * x[t] + beta * y[t];
+}
+
 // nu folded into the geometric factors: G_ij <- nu .* G_ij (semPS.jl:161-163)
 __global__ void k_vardiff(const double* __restrict__ nu, const double* __restrict__ G11,
                           const double* __restrict__ G12, const double* __restrict__ G22,
@@ -674,6 +693,21 @@ int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, cons
                double* part, int nparts, double* out) {
   hipLaunchKernelGGL(k_dot_part, dim3(nparts), dim3(256), 0, st, a, b, w, (long)n, part);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_gprod(hipStream_t st, int64_t n, const double* G11, const double* G12,
+                 const double* G22, const double* a, const double* b, double* wr, double* ws) {
+  hipLaunchKernelGGL(k_gprod, dim3(nblk(n)), dim3(256), 0, st, G11, G12, G22, a, b, wr, ws,
+                     (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta,
+                 const double* y, double* out) {
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, st, alpha, x, beta, y, out, (long)n);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

@@ -186,6 +186,11 @@ int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, cons
                double* part, int nparts, double* out);
 int launch_vardiff(hipStream_t st, const sem_mesh* m, const double* nu, double* G11, double* G12,
                    double* G22);
+int launch_gprod(hipStream_t st, int64_t n, const double* G11, const double* G12,
+                 const double* G22, const double* a, const double* b, double* wr, double* ws);
+int launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta,
+                 const double* y, double* out);
+
 
 int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* y);
 int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y);

@@ -664,6 +664,18 @@ int sem_combine(sem_ctx* ctx, const sem_mesh* m, int nterms, const double* coef,
   return SEM_OK;
 }
 
+int sem_gprod(sem_ctx* ctx, int64_t n, const double* G11, const double* G12, const double* G22,
+              const double* a, const double* b, double* wr, double* ws) {
+  SEM_ARG(ctx && G11 && G12 && G22 && a && b && wr && ws && n >= 0, "bad arguments");
+  return launch_gprod(ctx->stream, n, G11, G12, G22, a, b, wr, ws);
+}
+
+int sem_axpby(sem_ctx* ctx, int64_t n, double alpha, const double* x, double beta,
+              const double* y, double* out) {
+  SEM_ARG(ctx && x && y && out && n >= 0, "bad arguments");
+  return launch_axpby(ctx->stream, n, alpha, x, beta, y, out);
+}
+
 int sem_vardiff_factors(sem_ctx* ctx, const sem_mesh* m, const double* nu, double* G11,
                         double* G12, double* G22) {
   SEM_ARG(ctx && m && nu && G11 && G12 && G22, "NULL argument");

@@ -92,6 +92,8 @@ SIGNATURES = {
     "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],
     "sem_combine": [_vp, _vp, _i, _dp, _pvp, C.POINTER(C.c_int), _i, _i, _vp],
     "sem_vardiff_factors": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "sem_gprod": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sem_axpby": [_vp, _i64, _d, _vp, _d, _vp, _vp],
     "sem_grad": [_vp, _vp, _vp, _vp, _vp],
     "sem_grad_t": [_vp, _vp, _vp, _vp, _vp],
     "sem_adjoint_dnu": [_vp, _vp, _vp, _vp, _vp],

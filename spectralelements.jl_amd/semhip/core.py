@@ -347,24 +347,97 @@ def ABu(As, Br, u, trans: bool = False):
 
 
 def lapl(u, *args, out=None):
-    """lapl(u, msh) (lapl.jl:26-36) or lapl(u, nu, msh) = nu .* lapl(u, msh) (:38-45)."""
+    """lapl(u, msh) (lapl.jl:26-36); lapl(u, nu, msh) = nu .* lapl(u, msh)
+    (:38-45); lapl(u, msh1, msh2), whose dealiasing is a placeholder in the
+    reference and forwards to lapl(u, msh1) (:47-52); and the 12-argument
+    lapl(u, M, Jr, Js, QQtx, QQty, Dr, Ds, G11, G12, G22, mult) with
+    gather-scatter and mask (:54-68; Jr = Js = [] is the plain laplace, else
+    the dealiased laplace of :83-103)."""
+    if len(args) == 11:
+        return _lapl12(u, *args, out=out)
     if len(args) == 1:
+        nu, msh = None, args[0]
+    elif len(args) == 2 and isinstance(args[0], Mesh) and isinstance(args[1], Mesh):
         nu, msh = None, args[0]
     elif len(args) == 2:
         nu, msh = args
     else:
-        raise TypeError("lapl(u, msh) or lapl(u, nu, msh)")
+        raise TypeError("lapl(u, msh), lapl(u, nu, msh), lapl(u, msh1, msh2) or the "
+                        "12-argument form")
     if nu is None:
         return hlmz(u, 1.0, 0.0, msh, out=out)
     return hlmz(u, nu, 0.0, msh, out=out)
 
 
-def mass(u, msh: Mesh, out=None):
-    """B .* u (mass.jl:12-22)."""
+def _is_empty(A) -> bool:
+    return A is None or (hasattr(A, "__len__") and len(A) == 0) or (
+        hasattr(A, "numel") and A.numel() == 0) or (hasattr(A, "size") and not callable(A.size)
+                                                    and A.size == 0)
+
+
+def gprod(G11, G12, G22, a, b):
+    """(G11 .* a + G12 .* b, G12 .* a + G22 .* b) (lapl.jl:75-76)."""
+    a, b = to_device(a), to_device(b)
+    G = [to_device(g) for g in (G11, G12, G22)]
+    wr, ws = _like(a), _like(a)
+    L.call("sem_gprod", ctx(), a.numel(), _p(G[0]), _p(G[1]), _p(G[2]), _p(a), _p(b), _p(wr),
+           _p(ws))
+    return wr, ws
+
+
+def axpby(alpha, x, beta, y, out=None):
+    """alpha .* x .+ beta .* y."""
+    x, y = to_device(x), to_device(y)
+    o = out if out is not None else _like(x)
+    L.call("sem_axpby", ctx(), x.numel(), float(alpha), _p(x), float(beta), _p(y), _p(o))
+    return o
+
+
+def _lapl12(u, M, Jr, Js, QQtx, QQty, Dr, Ds, G11, G12, G22, mult, out=None):
+    """lapl.jl:54-68 from library calls: ABu derivatives (and interpolation),
+    the pointwise G product, the transposed contractions, QQ' through the map
+    built from the 0/1 matrices, and the mask. (The Zygote hook multiplying the
+    cotangent by mult is the identity in the forward pass.)"""
+    was_np = isinstance(u, np.ndarray)
+    u = to_device(u)
+    ur = ABu(None, Dr, u)
+    us = ABu(Ds, None, u)
+    deal = not (_is_empty(Jr) and _is_empty(Js))
+    if deal:
+        ur, us = ABu(Js, Jr, ur), ABu(Js, Jr, us)
+    vr, vs = gprod(G11, G12, G22, ur, us)
+    if deal:
+        vr, vs = ABu(Js, Jr, vr, trans=True), ABu(Js, Jr, vs, trans=True)
+    Au = axpby(1.0, ABu(None, Dr, vr, trans=True), 1.0, ABu(Ds, None, vs, trans=True))
+    Au = gatherScatter(Au, QQtx, QQty)
+    y = mask(Au, M, out=out)
+    return _wrap_out(y, was_np)
+
+
+def mass(u, *args, out=None):
+    """B .* u: mass(u, msh) (mass.jl:12-22); mass(u, msh1, msh2), which
+    forwards to mass(u, msh1) in the reference (:25-30); and the 8-argument
+    mass(u, M, B, Jr, Js, QQtx, QQty, mult) = mask(QQ'(Js'Jr'(B .* Js Jr u)))
+    (:32-50; B = [] means no weight)."""
+    if len(args) == 7:
+        return _mass8(u, *args, out=out)
+    if len(args) not in (1, 2) or not all(isinstance(a, Mesh) for a in args):
+        raise TypeError("mass(u, msh), mass(u, msh1, msh2) or the 8-argument form")
+    msh = args[0]
     was_np = isinstance(u, np.ndarray)
     u = to_device(u)
     y = out if out is not None else _like(u)
     L.call("sem_mass", ctx(), msh.h, _p(u), _p(y))
+    return _wrap_out(y, was_np)
+
+
+def _mass8(u, M, B, Jr, Js, QQtx, QQty, mult, out=None):
+    was_np = isinstance(u, np.ndarray)
+    Ju = ABu(Js, Jr, to_device(u))
+    BJu = Ju if _is_empty(B) else mask(Ju, to_device(B))
+    Bu = ABu(Js, Jr, BJu, trans=True)
+    Bu = gatherScatter(Bu, QQtx, QQty)
+    y = mask(Bu, M, out=out)
     return _wrap_out(y, was_np)
 
 
@@ -376,8 +449,10 @@ def _scalar_or_field(v):
     return 0.0, to_device(v)
 
 
-def hlmz(u, nu, k, msh: Mesh, out=None):
-    """nu .* lapl(u) + k .* mass(u) (hlmz.jl:12-19); nu, k scalars or fields."""
+def hlmz(u, nu, k, msh: Mesh, msh2: Optional[Mesh] = None, out=None):
+    """nu .* lapl(u) + k .* mass(u) (hlmz.jl:12-19); nu, k scalars or fields.
+    hlmz(u, nu, k, msh1, msh2) forwards to the first mesh, as the reference's
+    placeholder for dealiasing does (hlmz.jl:22-30)."""
     was_np = isinstance(u, np.ndarray)
     u = to_device(u)
     y = out if out is not None else _like(u)

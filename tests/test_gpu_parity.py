@@ -466,3 +466,35 @@ def test_p16_fp64_and_fp32_contraction_paths(se, rows_env, fma16):
             os.environ.pop("SEM_AX_FMA16", None)
         else:
             os.environ["SEM_AX_FMA16"] = old
+
+
+@pytest.mark.parametrize("dealiased", [False, True])
+def test_twelve_argument_lapl_and_eight_argument_mass(se, dealiased):
+    """lapl(u, M, Jr, Js, QQtx, QQty, Dr, Ds, G11, G12, G22, mult) (lapl.jl:54-68,
+    83-103) and mass(u, M, B, Jr, Js, QQtx, QQty, mult) (mass.jl:32-50) through
+    the library against the oracle, with the dense 0/1 QQ factors of the
+    reference; the two-mesh forms forward to the first mesh (lapl.jl:47-52)."""
+    _log(f"12-arg lapl / 8-arg mass dealiased={dealiased}")
+    mo = o.Mesh(5, 5, 3, 2, deform=o.deform_sinsin(0.1), dense_gs=True)
+    Mo = o.generateMask("DNDD", mo)
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(mo.x.shape)
+    if dealiased:
+        md = o.Mesh(8, 8, 3, 2, deform=o.deform_sinsin(0.1), dense_gs=False)
+        Jr, Js = o.interpMat(md.zr, mo.zr), o.interpMat(md.zs, mo.zs)
+        G = [rng.standard_normal(md.x.shape) for _ in range(3)]
+        Bd = rng.standard_normal(md.x.shape)
+    else:
+        Jr = Js = np.zeros((0, 0))
+        G = [mo.G11, mo.G12, mo.G22]
+        Bd = mo.B
+    ref = o.lapl12(u, Mo, Jr, Js, mo.QQtx, mo.QQty, mo.Dr, mo.Ds, *G, mo.mult)
+    got = se.lapl(u, Mo, Jr, Js, mo.QQtx, mo.QQty, mo.Dr, mo.Ds, *G, mo.mult)
+    assert _rel(got, ref) < 1e-12
+    ref = o.mass8(u, Mo, Bd, Jr, Js, mo.QQtx, mo.QQty, mo.mult)
+    got = se.mass(u, Mo, Bd, Jr, Js, mo.QQtx, mo.QQty, mo.mult)
+    assert _rel(got, ref) < 1e-12
+    m = se.Mesh(5, 5, 3, 2, (False, False), se.deform_sinsin(0.1))
+    np.testing.assert_array_equal(se.lapl(u, m, m), se.lapl(u, m))
+    np.testing.assert_array_equal(se.mass(u, m, m), se.mass(u, m))
+    np.testing.assert_array_equal(se.hlmz(u, 1.2, 0.3, m, m), se.hlmz(u, 1.2, 0.3, m))
