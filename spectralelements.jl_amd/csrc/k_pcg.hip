@@ -397,9 +397,12 @@ struct PcgRun {
 
   // launches iterations k_start.. until the stop test fires (norm(r, Inf) <=
   // tol or k - 1 == maxiter); on return hs holds the final state
-  int run(T* x, double tol, int maxiter, int k_start) {
+  // hint: a fresh solve starts with a batch the size of this operator's last
+  // converged solve (repeated solves: time steps, adjoints, FD) and records it
+  int run(T* x, double tol, int maxiter, int k_start, bool hint = false) {
     hipStream_t st = ctx->stream;
-    int k = k_start, batch = 4;
+    int k = k_start,
+        batch = (hint && op->last_iters > 0) ? std::min(op->last_iters + 2, 1024) : 4;
     const long kmax = (long)maxiter + 1;
     for (;;) {
       for (int q = 0; q < batch && k <= kmax; ++q, ++k) {
@@ -420,7 +423,10 @@ struct PcgRun {
       SEM_HIP(hipGetLastError());
       SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
       SEM_HIP(hipStreamSynchronize(st));
-      if (hs->done) return SEM_OK;
+      if (hs->done) {
+        if (hint && hs->converged) op->last_iters = hs->iters;
+        return SEM_OK;
+      }
       if (k > kmax) {
         set_error("pcg: iteration limit passed without the stop test firing");
         return SEM_ERR_BREAKDOWN;
@@ -438,7 +444,7 @@ int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x
   hipStream_t st = ctx->stream;
   SEM_HIP(hipEventRecord(ctx->ev0, st));
   SEM_TRY(R.init(b, x));
-  SEM_TRY(R.run(x, tol, maxiter, 1));
+  SEM_TRY(R.run(x, tol, maxiter, 1, true));
   SEM_HIP(hipEventRecord(ctx->ev1, st));
   SEM_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;

@@ -80,6 +80,13 @@ struct sem_mesh {
   std::vector<double> hDr, hDs, hzr, hwr, hzs, hws;
   int32_t* l2g = nullptr;
   sem_gsmap* map = nullptr;
+  // interpolation matrices [Jr | Js] from a (nr, ns)-point GLL grid onto this
+  // mesh's nodes (advect.jl:66-78), uploaded once per source grid
+  struct Interp {
+    int nr, ns;
+    double* J;
+  };
+  mutable std::vector<Interp> interp;
   sem::Layout layout() const { return sem::Layout{nr, Ex, Ey, L, NC, per_x, per_y}; }
 };
 
@@ -98,6 +105,9 @@ struct sem_operator {
   bool has_mass = false;
   // MFMA operand tables (FP16/BF16 bit patterns) for the contraction matrices
   void* Dmfma = nullptr;
+  // iterations of the last fresh solve with this operator: the first launch
+  // batch of the next one (repeated solves: time steps, adjoints, FD)
+  mutable int last_iters = 0;
 };
 
 namespace sem {

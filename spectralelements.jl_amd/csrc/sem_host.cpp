@@ -188,6 +188,7 @@ void free_mesh(sem_mesh* m) {
     hipFree(m->map->seg);
     delete m->map;
   }
+  for (auto& it : m->interp) hipFree(it.J);
   delete m;
 }
 
@@ -716,15 +717,22 @@ static int advect_common(sem_ctx* ctx, const sem_mesh* mV, const sem_mesh* mD, c
   SEM_ARG(mV->nr == mV->ns && mD->nr == mD->ns, "advect: needs nr == ns on both meshes");
   SEM_ARG(mV->nr <= kMaxP && mD->nr <= 24, "advect: supports p <= 16 and dealiased p <= 24");
   const int p = mV->nr, pd = mD->nr;
-  std::vector<double> J(2 * (size_t)pd * p);
-  interpmat(pd, mD->hzr.data(), p, mV->hzr.data(), J.data());
-  interpmat(pd, mD->hzs.data(), p, mV->hzs.data(), J.data() + (size_t)pd * p);
-  SEM_TRY(ensure_ws(ctx, sizeof(double) * J.size()));
-  double* dJ = static_cast<double*>(ctx->ws);
-  SEM_HIP(hipMemcpyAsync(dJ, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice,
-                         ctx->stream));
+  double* dJ = nullptr;
+  for (auto& it : mD->interp)
+    if (it.nr == mV->nr && it.ns == mV->ns) dJ = it.J;
+  if (!dJ) {  // first use of this source grid: build and upload once
+    std::vector<double> J(2 * (size_t)pd * p);
+    interpmat(pd, mD->hzr.data(), p, mV->hzr.data(), J.data());
+    interpmat(pd, mD->hzs.data(), p, mV->hzs.data(), J.data() + (size_t)pd * p);
+    SEM_HIP(hipMalloc(&dJ, sizeof(double) * J.size()));
+    if (hipMemcpy(dJ, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      hipFree(dJ);
+      set_error("advect: upload of the interpolation matrices failed");
+      return SEM_ERR_HIP;
+    }
+    mD->interp.push_back({mV->nr, mV->ns, dJ});
+  }
   SEM_TRY(launch_advect(ctx->stream, mV, mD, dJ, T, vx, vy, y, transpose));
-  SEM_HIP(hipStreamSynchronize(ctx->stream));  // J lives on this call's stack
   return SEM_OK;
 }
 
