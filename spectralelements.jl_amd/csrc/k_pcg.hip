@@ -109,7 +109,7 @@ constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP
 // HAS_D: a diagonal preconditioner is present. FULL: L is a multiple of
 // kUpdThreads and the row table fits, so the bulk row loop has a uniform trip
 // count (scalar loop control, no per-lane exit masks).
-template <typename T, bool HAS_D, bool FULL>
+template <typename T, bool HAS_D, bool FULL, int VEC = 1>
 __global__ void __launch_bounds__(kUpdThreads)
 k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
              const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
@@ -151,7 +151,56 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     }
     const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
     const long cj = L * j;
-    if constexpr (FULL) {
+    if constexpr (FULL && VEC == 2) {
+      // two adjacent rows per thread with 8-byte accesses (FP32)
+      static_assert(sizeof(T) == 4, "paired rows: FP32");
+      using T2 = float2;
+      const bool yseam = jL >= 0;
+      const long cL = L * (yseam ? jL : 0), cR = L * (yseam ? jR : 0);
+#pragma unroll 2
+      for (int i0 = 0; i0 < Lr; i0 += 2 * kUpdThreads) {
+        const int i = i0 + 2 * (int)threadIdx.x;
+        const long t = i + cj;
+        const int2 info = *reinterpret_cast<const int2*>(rinfo + i);
+        T2 wf;
+        if (yseam) {
+          const T2 wl = *reinterpret_cast<const T2*>(w + i + cL);
+          const T2 wr2 = *reinterpret_cast<const T2*>(w + i + cR);
+          const T m0 = ((info.x & 8) || masky) ? T(0) : T(1);
+          const T m1 = ((info.y & 8) || masky) ? T(0) : T(1);
+          wf.x = (wl.x + wr2.x) * m0;
+          wf.y = (wl.y + wr2.y) * m1;
+        } else {
+          wf = *reinterpret_cast<const T2*>(w + t);
+        }
+        const T2 xo = *reinterpret_cast<const T2*>(x + t);
+        const T2 ro = *reinterpret_cast<const T2*>(r + t);
+        const T2 pv = *reinterpret_cast<const T2*>(p + t);
+        const bool k0 = !(info.x & 6), k1 = !(info.y & 6);
+        T2 xn, rn;
+        xn.x = xo.x + alpha * pv.x;
+        xn.y = xo.y + alpha * pv.y;
+        rn.x = ro.x - alpha * wf.x;
+        rn.y = ro.y - alpha * wf.y;
+        T2 xs, rs;
+        xs.x = k0 ? xn.x : xo.x;
+        xs.y = k1 ? xn.y : xo.y;
+        rs.x = k0 ? rn.x : ro.x;
+        rs.y = k1 ? rn.y : ro.y;
+        *reinterpret_cast<T2*>(x + t) = xs;
+        *reinterpret_cast<T2*>(r + t) = rs;
+        T2 z = rn;
+        if constexpr (HAS_D) {
+          const T2 dv = *reinterpret_cast<const T2*>(d + t);
+          z.x = dv.x * rn.x;
+          z.y = dv.y * rn.y;
+        }
+        rho += (double)rn.x * (double)z.x * (k0 ? (info.x & 1 ? 0.5 : 1.0) * my : 0.0);
+        rho += (double)rn.y * (double)z.y * (k1 ? (info.y & 1 ? 0.5 : 1.0) * my : 0.0);
+        mx = fmax(mx, k0 ? fabs((double)rn.x) : 0.0);
+        mx = fmax(mx, k1 ? fabs((double)rn.y) : 0.0);
+      }
+    } else if constexpr (FULL) {
       if (jL >= 0) {   // uniform: a segment y-seam column
         const long cL = L * jL, cR = L * jR;
 #pragma unroll 2
@@ -381,11 +430,21 @@ struct PcgRun {
   void launch_update(hipStream_t st, T* x, int k) {
     const size_t lds = g.L <= kRowTable ? sizeof(int) * g.L : 0;
     const bool full = g.L % kUpdThreads == 0 && g.L <= kRowTable;
+    // FP32 with rows in pairs: L a multiple of 2 kUpdThreads (8-byte aligned
+    // pairs, since every column starts at an even offset)
+    const bool pairs = sizeof(T) == 4 && g.L % (2 * kUpdThreads) == 0 && g.L <= kRowTable;
     double* pk = part + 2 * n3 * (k & 1);
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, x, r, p, w, d, Q, nQ, state,
                          k, pk, n3, sg, n);
     };
+    if constexpr (sizeof(T) == 4) {
+      if (pairs) {
+        if (d) go(k_pcg_update<T, true, true, 2>);
+        else go(k_pcg_update<T, false, true, 2>);
+        return;
+      }
+    }
     if (d) {
       if (full) go(k_pcg_update<T, true, true>);
       else go(k_pcg_update<T, true, false>);

@@ -498,3 +498,51 @@ def test_twelve_argument_lapl_and_eight_argument_mass(se, dealiased):
     np.testing.assert_array_equal(se.lapl(u, m, m), se.lapl(u, m))
     np.testing.assert_array_equal(se.mass(u, m, m), se.mass(u, m))
     np.testing.assert_array_equal(se.hlmz(u, 1.2, 0.3, m, m), se.hlmz(u, 1.2, 0.3, m))
+
+
+def test_fused_pcg_and_mixed_solves_are_deterministic(se):
+    """Fixed-order reductions everywhere: repeated solves (FP64 fused PCG and
+    the FP32 mixed solver) give bit-identical solutions and iteration counts."""
+    import torch
+    _log("solve determinism")
+    m = se.Mesh(8, 8, 24, 24, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    b = se.gatherScatter(se.mask(se.mass(se.workloads.global_normal_field(m, 4), m), M), m)
+    d = op.jacobi()
+    tol = 1e-10 * float(b.abs().max())
+    runs = []
+    for _ in range(3):
+        info = se.PCGInfo()
+        x = se.pcg(b, op, opM=se.Diagonal(d), mult=m.mult, tol=tol, info=info)
+        runs.append((x.clone(), info.iters))
+    assert all(torch.equal(r[0], runs[0][0]) and r[1] == runs[0][1] for r in runs)
+    op32 = se.SEMOperator(1.0, 1.0, m, M, precision="f32")
+    xs = [se.pcg_mixed(b, op, op32, d=d, tol=tol, inner_tol=1e-3) for _ in range(2)]
+    assert torch.equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("n,Ex", [(16, 128), (8, 256)])
+def test_fp32_solves_with_paired_row_update(se, n, Ex):
+    """Meshes whose local row count is a multiple of 2048 take the FP32 update
+    kernel that handles rows in pairs: an FP32 fused PCG and the mixed solver
+    on such a mesh against the FP64 solve."""
+    _log(f"paired-row FP32 update n={n} Ex={Ex}")
+    m = se.Mesh(n, n, Ex, 4, (False, False), se.deform_sinsin(0.1))
+    assert m.shape[0] % 2048 == 0
+    M = se.generateMask("DDDD", m)
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    op32 = se.SEMOperator(1.0, 1.0, m, M, precision="f32")
+    b = se.gatherScatter(se.mask(se.mass(se.workloads.global_normal_field(m, 6), m), M), m)
+    d = op64.jacobi()
+    tol = 1e-10 * float(b.abs().max())
+    x64 = se.to_host(se.pcg(b, op64, opM=se.Diagonal(d), mult=m.mult, tol=tol))
+    info = se.PCGInfo()
+    x32 = se.pcg(b.float(), op32, opM=se.Diagonal(d.float()), mult=m.mult,
+                 tol=1e-5 * float(b.abs().max()), info=info)
+    assert info.converged
+    assert _rel(se.to_host(x32.double()), x64) < 1e-4
+    info = se.PCGInfo()
+    xm = se.to_host(se.pcg_mixed(b, op64, op32, d=d, tol=tol, inner_tol=1e-3, info=info))
+    assert info.converged
+    assert _rel(xm, x64) < 1e-8
