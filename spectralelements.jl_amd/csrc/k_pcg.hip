@@ -613,13 +613,15 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
   bool reliable = menv ? std::atoi(menv) != 0
                        : (op_lo->precision == SEM_F32 || op_lo->precision == SEM_F16X3_MFMA ||
                           op_lo->precision == SEM_BF16X3_MFMA);
-  hipEvent_t e0, e1;
-  SEM_HIP(hipEventCreate(&e0));
-  SEM_HIP(hipEventCreate(&e1));
-  SEM_HIP(hipEventRecord(e0, st));
-  SEM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
-  SEM_HIP(hipMemsetAsync(x32, 0, sizeof(float) * n, st));
-  SEM_HIP(hipMemcpyAsync(r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;   // the context's timing events
+  if (hipEventRecord(e0, st) != hipSuccess ||
+      hipMemsetAsync(x, 0, sizeof(double) * n, st) != hipSuccess ||
+      hipMemsetAsync(x32, 0, sizeof(float) * n, st) != hipSuccess ||
+      hipMemcpyAsync(r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+    set_error("sem_pcg_mixed: HIP call failed during set-up");
+    cleanup();
+    return SEM_ERR_HIP;
+  }
   double* hres = reinterpret_cast<double*>(static_cast<char*>(ctx->pinned) + 64);
   auto true_norm = [&](double* out) -> int {
     SEM_TRY(launch_norm_inf(st, n, r64, nrm, 256, nrm + 256));
@@ -664,7 +666,11 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
       k = R.hs->iters + 1;
       total += k - k0;
       const double inner_res = R.hs->res;
-      SEM_HIP(hipMemcpyAsync(xb, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      if (hipMemcpyAsync(xb, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        set_error("sem_pcg_mixed: copy failed");
+        s = SEM_ERR_HIP;
+        break;
+      }
       hipLaunchKernelGGL(k_accum_zero, dim3(nblk(n)), dim3(256), 0, st, x, x32, scale, n);
       const double res_prev = res;
       if ((s = true_residual(&res)) != SEM_OK) break;
@@ -679,7 +685,11 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
       const bool bad = !(res < 10.0 * res_prev) || !std::isfinite(inner_res);
       if (bad) {
         // roll back to the last accepted iterate, restart the Krylov space
-        SEM_HIP(hipMemcpyAsync(x, xb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if (hipMemcpyAsync(x, xb, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+          set_error("sem_pcg_mixed: copy failed");
+          s = SEM_ERR_HIP;
+          break;
+        }
         if ((s = true_residual(&res)) != SEM_OK) break;
         ++restarts;
       }
@@ -697,18 +707,22 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
         hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, st, r64, R.r, 1.0 / ns, n);
         hipLaunchKernelGGL(k_rescale, dim3(nblk(n)), dim3(256), 0, st, R.p,
                            (float)(scale / ns), R.state, k, n);
-        SEM_HIP(hipGetLastError());
+        if (hipGetLastError() != hipSuccess) {
+          set_error("sem_pcg_mixed: kernel launch failed");
+          s = SEM_ERR_HIP;
+          break;
+        }
         scale = ns;
         s = R.reprime(k);
       }
     }
   }
-  SEM_HIP(hipEventRecord(e1, st));
-  SEM_HIP(hipEventSynchronize(e1));
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (s == SEM_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) {
+    set_error("sem_pcg_mixed: timing event failed");
+    s = SEM_ERR_HIP;
+  }
+  if (s == SEM_OK) (void)hipEventElapsedTime(&ms, e0, e1);
   cleanup();
   if (s != SEM_OK) return s;
   if (!std::isfinite(res)) status = SEM_ERR_NONFINITE;
