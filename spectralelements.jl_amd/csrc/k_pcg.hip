@@ -120,14 +120,18 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   const long L = sg.L;
   __shared__ double red[kUpdThreads];
   extern __shared__ int rinfo[];
+  // independent loads first, so their latencies overlap: this thread's share
+  // of the <p, Ap> partials and t = <r, z> of this iteration (both valid
+  // memory even when the solve has already stopped)
+  double qs = 0.0;
+  for (int q = threadIdx.x; q < nQ; q += kUpdThreads) qs += Q[q];
+  const double tk = st->t[k & 1];
   if (block_done(st, red)) return;
   const bool table = FULL || Lr <= kRowTable;   // uniform
   if (table)
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
-  double qs = 0.0;
-  for (int q = threadIdx.x; q < nQ; q += kUpdThreads) qs += Q[q];
   const double pw = block_sum_w<kUpdThreads>(qs, red);   // synchronises the block
-  const T alpha = T(st->t[k & 1] / pw);
+  const T alpha = T(tk / pw);
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
   const int j0 = blockIdx.x * cpb;
   const int j1 = min(NC, j0 + cpb);
