@@ -26,6 +26,8 @@
 //    k_seams in the same order. Every node copy is finalised exactly once.
 //  * No atomics; the result is deterministic.
 #pragma once
+#include <algorithm>
+
 #include "sem_device.h"
 #include "_gen/gll_dtables.h"
 
@@ -575,19 +577,41 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
 #undef Ds
 }
 
-// Finishes the node copies the strips could not sum: x seams (strip
-// boundaries and the periodic x wrap), then y seams (segment boundaries and
-// the periodic y wrap). One item per seam row/column entry; items touch
-// disjoint copies, so there are no races.
+// Finishes the node copies the strips could not sum: y seams (segment
+// boundaries and the periodic y wrap), then x seams (strip boundaries and the
+// periodic x wrap). Grid row blockIdx.y is one seam line: y seam g < nys
+// (threads over the L rows), else x seam s = blockIdx.y - nys (threads over
+// the NC columns); no per-thread integer division. Items touch disjoint
+// copies, so there are no races.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs,
-        int per_x, int per_y, int bc, long nxitems, long total) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        int per_x, int per_y, int bc, int nys) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   auto fin = [&](int i, int j, T s) { y[i + (long)L * j] = s * mval<T>(i, j, L, NC, bc); };
-  if (t < nxitems) {
-    const int s = (int)(t / NC);
-    const int j = (int)(t - (long)s * NC);
+  if ((int)blockIdx.y < nys) {
+    const int g = blockIdx.y;
+    const int i = q;
+    if (i >= L) return;
+    const int r = i % CXP;
+    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
+                        (per_x && (i == 0 || i == L - 1));
+    if (xs_row) return;   // summed with the four-copy rule by the x-seam items
+    int jL, jR;
+    if (g < nsegs - 1) {
+      jR = (g + 1) * SP;
+      jL = jR - 1;
+    } else {
+      jL = NC - 1;
+      jR = 0;
+    }
+    const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
+    fin(i, jL, sm);
+    fin(i, jR, sm);
+  } else {
+    const int s = blockIdx.y - nys;
+    const int j = q;
+    if (j >= NC) return;
     int iL, iR;
     if (s < nstrips - 1) {
       iR = (s + 1) * CXP;
@@ -612,25 +636,6 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
       fin(iL, j, sm);
       fin(iR, j, sm);
     }
-  } else if (t < total) {
-    const long t2 = t - nxitems;
-    const int g = (int)(t2 / L);
-    const int i = (int)(t2 - (long)g * L);
-    const int r = i % CXP;
-    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
-                        (per_x && (i == 0 || i == L - 1));
-    if (xs_row) return;
-    int jL, jR;
-    if (g < nsegs - 1) {
-      jR = (g + 1) * SP;
-      jL = jR - 1;
-    } else {
-      jL = NC - 1;
-      jR = 0;
-    }
-    const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
-    fin(i, jL, sm);
-    fin(i, jR, sm);
   }
 }
 
@@ -739,9 +744,11 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   }
   // In PCG mode the seam copies are finished by the update kernel instead.
   if (a.assemble && pl.nk2 > 0 && !pc) {
-    hipLaunchKernelGGL((k_seams<T>), dim3(pl.nk2), dim3(256), 0, st, prm.y, g.L, g.NC, P, CX * P,
-                       pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, pl.nxitems,
-                       pl.total);
+    const int nys = pl.nsegs - 1 + (g.per_y ? 1 : 0);
+    const int nxs = pl.nstrips - 1 + (g.per_x ? 1 : 0);
+    const dim3 sgrid((std::max(g.L, g.NC) + 255) / 256, nys + nxs);
+    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(256), 0, st, prm.y, g.L, g.NC, P, CX * P,
+                       pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys);
   }
   SEM_HIP(hipGetLastError());
   return SEM_OK;
