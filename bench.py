@@ -312,8 +312,9 @@ def main():
                     help="also run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the operator would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only; multi-GPU runs are not supported (the "
+              "operator would shard in y-strips of elements with one halo exchange per "
+              "apply and one 3-scalar allreduce per PCG iteration)", file=sys.stderr)
         sys.exit(2)
     import torch
     import semhip as se
