@@ -223,7 +223,7 @@ def bench_adjoint(se, n_dirs=20):
             worst = max(worst, abs(fd - ad) / abs(ad))
     return {"workload": "-div(nu grad u)=1, 128x128 elements, N=7, deformed, nu=1+0.5tanh(16 "
                         "Fourier modes, seed 41), u* from seed 42, J=1/2 sum B (u-u*)^2",
-            "J": float(J), "forward_s": t1 - t0, "adjoint_s": t2 - t1,
+            "J": float(J.detach()), "forward_s": t1 - t0, "adjoint_s": t2 - t1,
             "gradient_s": t2 - t0, "iters_forward": it_fwd, "iters_adjoint": it_adj,
             "tol_rel": 1e-13, "fd_dirs": n_dirs, "fd_h": 1e-5,
             "fd_max_rel_err": worst, "fd_target": 1e-6}

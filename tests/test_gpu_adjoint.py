@@ -63,7 +63,7 @@ def _linsolve_grad(se, m, M, nu_h, f_h, ustar, tol):
     e = u - ustar
     J = 0.5 * torch.sum(m.B * e * e)
     J.backward()
-    return float(J), u.detach(), nu.grad
+    return float(J.detach()), u.detach(), nu.grad
 
 
 def test_linsolve_gradient_matches_oracle(se):
