@@ -37,7 +37,7 @@ __all__ = [
     "deform_sinsin", "deform_affine01", "fixU", "local_to_global",
     "advect_transpose", "vardiff_operator", "vardiff_gradient", "cd_forward",
     "laplace_dealiased", "lapl12", "mass8",
-    "cd_adjoint", "cd_objective",
+    "cd_adjoint", "cd_objective", "annulus", "diffusion_simulate",
 ]
 
 
@@ -271,6 +271,16 @@ def deform_sinsin(amp: float = 0.1):
 def deform_affine01(x, y):
     """usrgeom of p2d_explicit.jl:78 / semPS.jl:306-310: [-1,1]^2 -> [0,1]^2."""
     return 0.5 * (x + 1.0), 0.5 * (y + 1.0)
+
+
+def annulus(r0: float = 0.5, r1: float = 1.0, span: float = 2.0 * math.pi):
+    """annulus(r, s; r0, r1, span) (geom.jl:40-50): R = (r1 - r0)/2 (r + 1) +
+    r0, th = span/2 (s + 1), (x, y) = (R cos th, R sin th)."""
+    def f(x, y):
+        R = (r1 - r0) / 2 * (x + 1) + r0
+        th = span / 2 * (y + 1) + 0.0
+        return R * np.cos(th), R * np.sin(th)
+    return f
 
 
 def local_to_global(Ex: int, Ey: int, nr: int, ns: int, periodic=(False, False)):
@@ -653,7 +663,7 @@ class TimeStepper:
     """time.jl:70-112.
 
     The reference constructor always calls bdfExtK(time) with its default k=3
-    (time.jl:358), so its k != 3 steppers fail at the first bdfExtK! broadcast;
+    (time.jl:87), so its k != 3 steppers fail at the first bdfExtK! broadcast;
     here the constructor uses the stepper's own k, which equals the reference
     for k = 3 and makes the BDF2/EXT2 (k = 2) stepper usable.
     """
@@ -805,3 +815,74 @@ def cd_adjoint(mV: Mesh, mD: Mesh, u0, vx, vy, nu, dt, nsteps, M, k=2, tol_rel=1
             if b[1 + i] != 0.0 or a[i] != 0.0:
                 acc[m] = acc[m] - b[1 + i] * (mV.B * lam) + a[i] * (-Ct)
     return J, acc[0]
+
+
+# --------------------------------------------------------------------------
+# the Diffusion equation (diffusion.jl)
+# --------------------------------------------------------------------------
+def diffusion_simulate(msh: Mesh, bc, Ti=0.0, Tf=0.0, dt=0.0, k=3, setIC=None, setBC=None,
+                       setForcing=None, setVisc=None, nsteps=None, tol=1e-8, maxiter=None,
+                       callback=None):
+    """simulate!(Diffusion(bc, msh; Ti, Tf, dt, k), callback!, setIC!, setBC!,
+    setForcing!, setVisc!) (diffusion.jl:20-34, :110-137).
+
+    Setters have the reference signature setter(array, x, y, t) and write in
+    place; None leaves the array as it is (fixU!). Each step (evolve!,
+    diffusion.jl:81-106): histories shift (mesh.jl:199-215), the stepper
+    advances (time.jl:101-112), ub, f and nu are set at the new time, rhs =
+    QQ' mask(B f - nu lapl(ub) - sum_i b_{1+i} B uh_i) (:51-65), u = pcg(rhs,
+    mask(QQ' hlmz(., nu, b_1))) with the identity preconditioner, mult
+    weights and the absolute tolerance tol (:36-49, :67-77, pcg.jl:16-60),
+    then u += ub. The loop is the reference's ``while time <= Tf`` with its
+    break after a steady step (time < 1e-12), or ``nsteps`` steps by count.
+    The field keeps k histories (the reference's Field always keeps 3, which
+    equals k for its default k = 3). Returns (u, per-step PCG iteration
+    counts); callback(u, t, istep) runs after the initial condition and after
+    every step."""
+    M = generateMask(bc, msh)
+    ts = TimeStepper(Ti, Tf, dt, k=k)
+    x, y = msh.x, msh.y
+    u = np.zeros_like(x)
+    uh = [np.zeros_like(x) for _ in range(k)]
+    ub = np.zeros_like(x)
+    f = np.zeros_like(x)
+    nu = np.zeros_like(x)
+    iters = []
+    if setIC is not None:
+        setIC(u, x, y, ts.time[0])
+    if callback is not None:
+        callback(u, ts.time[0], ts.istep)
+
+    def evolve():
+        nonlocal u
+        for i in range(k - 1, 0, -1):
+            uh[i] = uh[i - 1]
+        uh[0] = u.copy()
+        ts.update()
+        t = ts.time[0]
+        for setter, arr in ((setBC, ub), (setForcing, f), (setVisc, nu)):
+            if setter is not None:
+                setter(arr, x, y, t)
+        a, b = ts.bdfA, ts.bdfB
+        rhs = mass(f, msh)
+        rhs = rhs - nu * lapl(ub, msh)
+        for i in range(k):
+            rhs = rhs - b[1 + i] * mass(uh[i], msh)
+        rhs = msh.gs(mask(rhs, M))
+        info = PCGInfo()
+        u = pcg(rhs, lambda v: mask(msh.gs(hlmz(v, nu, b[0], msh)), M), mult=msh.mult,
+                tol=tol, maxiter=maxiter, info=info)
+        u = u + ub
+        iters.append(info.iters)
+        if callback is not None:
+            callback(u, ts.time[0], ts.istep)
+
+    if nsteps is not None:
+        for _ in range(int(nsteps)):
+            evolve()
+    else:
+        while ts.time[0] <= Tf:
+            evolve()
+            if ts.time[0] < 1e-12:
+                break
+    return u, iters

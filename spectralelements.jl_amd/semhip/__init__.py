@@ -10,7 +10,7 @@ from .core import (  # noqa: F401
     ABu, Diagonal, GSMap, Mesh, PCGInfo, SEMOperator, adjoint_dnu, advect, ctx, deform_affine01,
     deform_sinsin, derivMat, empty_field, fixU, gatherScatter, gausslobatto, generateMask, grad,
     gradT, hlmz, identity, interpMat, lapl, mask, mass, ndgrid, norm_inf, pcg, pcg_, pcg_mixed,
-    semmesh, synchronize, to_device, to_host, wdot, dot, combine, vardiff_factors,
+    semmesh, synchronize, to_device, to_host, wdot, dot, combine, vardiff_factors, annulus,
 )
 from .adjoint import (  # noqa: F401
     VarDiffusion, VarDiffusionOperator, diffusion_solver, linsolve, nodal_gradient,
@@ -18,4 +18,5 @@ from .adjoint import (  # noqa: F401
 from .cd import (  # noqa: F401
     CDGradient, ConvectionDiffusion, Field, TimeStepper, bdfExtK, cd_gradient, cd_objective,
 )
+from .diffusion import Diffusion  # noqa: F401
 from . import workloads  # noqa: F401

@@ -212,6 +212,20 @@ class _Affine01:
 deform_affine01 = _Affine01()
 
 
+class annulus:
+    """annulus(r, s; r0, r1, span) (geom.jl:40-50): R = (r1 - r0)/2 (r + 1) +
+    r0, th = span/2 (s + 1), (x, y) = (R cos th, R sin th). A user
+    deformation: evaluated in FP64 on the device, then handed to the mesh."""
+
+    def __init__(self, r0: float = 0.5, r1: float = 1.0, span: float = 2.0 * np.pi):
+        self.r0, self.r1, self.span = float(r0), float(r1), float(span)
+
+    def __call__(self, x, y):
+        R = (self.r1 - self.r0) / 2 * (x + 1) + self.r0
+        th = self.span / 2 * (y + 1) + 0.0
+        return R * torch.cos(th), R * torch.sin(th)
+
+
 # -------------------------------------------------------------------- mesh
 class Mesh:
     """Mesh(nr, ns, Ex, Ey, ifperiodic=[false,false], deform=fixU), mesh.jl:66-133.
