@@ -40,6 +40,9 @@
 #ifndef SEM_AX_EARLYG_BYTES
 #define SEM_AX_EARLYG_BYTES 80
 #endif
+#ifndef SEM_AX_INPLACE
+#define SEM_AX_INPLACE 1
+#endif
 
 namespace sem {
 namespace ax {
@@ -164,10 +167,16 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   static_assert(MF == 0 || (P == 16 && NR == 256), "MFMA path: 16x16 blocks");
   static_assert(MF == 0 || MF == 6 || sizeof(T) == 4, "MFMA modes 1..5 take FP32 fields");
   static_assert(MF != 6 || sizeof(T) == 8, "MFMA mode 6 takes FP64 fields");
+  // The FMA path (MF == 0) runs C1, R2 and C2 in place: in each of them a
+  // thread reads a set of s0 slots and then rewrites exactly that set (C1/C2:
+  // column a of element e; R2: row tid), so s1 aliases s0 there and the LDS
+  // per workgroup drops by P * RS values. SEM_AX_INPLACE=0 keeps two buffers.
+  constexpr bool kInPlace = MF == 0 && SEM_AX_INPLACE;
   __shared__ T s0[P * RS];
-  __shared__ T s1[P * RS];
-  // x-pair exchange buffer of the MFMA path (R3 reads s0 and s1 there)
-  __shared__ T xbuf[MF ? 2 * NR : 1];
+  __shared__ T s1_[kInPlace ? 1 : P * RS];
+  T* const s1 = kInPlace ? s0 : s1_;
+  // x-pair exchange buffer (R3 reads s1, which is s0 on the in-place path)
+  __shared__ T xbuf[(MF || kInPlace) ? 2 * NR : 1];
   __shared__ double red[NR];
   // MFMA A-operands: D and D' (Dr = Ds); hi and lo parts (MF 1..4) or the
   // four K-blocks (MF 5, 6)
@@ -504,9 +513,10 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
       if constexpr (MF != 0) __syncthreads();  // s0, s1 were read in R3
       continue;
     }
-    // x-pairs inside the strip through LDS (s0 was last read in C2; on the
-    // MFMA path R3 reads s0 and s1, so the exchange has its own buffer)
-    T* xb = MF ? xbuf : s0;
+    // x-pairs inside the strip through LDS: s0 on the two-buffer FMA path
+    // (s0 was last read in C2), else the exchange buffer (R3 reads s1, which
+    // is s0 in place, and both buffers on the MFMA path)
+    T* xb = (MF || kInPlace) ? xbuf : s0;
     if (xside >= 0) {
 #pragma unroll
       for (int b = 0; b < P; ++b) xb[(xside * CX + e) * P + b] = h[b];
