@@ -591,22 +591,19 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
 // boundaries and the periodic y wrap), then x seams (strip boundaries and the
 // periodic x wrap). Grid row blockIdx.y is one seam line: y seam g < nys
 // (threads over the L rows), else x seam s = blockIdx.y - nys (threads over
-// the NC columns); no per-thread integer division. Items touch disjoint
-// copies, so there are no races.
+// the NC columns); the line's operands are uniform per block and there is
+// no per-thread integer division. Items touch disjoint copies, so there are
+// no races. The kernel is latency-bound (a few MB in thousands of short
+// blocks): 128-thread blocks measured best (1 item per thread beat 4).
+constexpr int kSeamBlock = 128;
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSeamBlock)
 k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs,
         int per_x, int per_y, int bc, int nys) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   auto fin = [&](int i, int j, T s) { y[i + (long)L * j] = s * mval<T>(i, j, L, NC, bc); };
   if ((int)blockIdx.y < nys) {
     const int g = blockIdx.y;
-    const int i = q;
-    if (i >= L) return;
-    const int r = i % CXP;
-    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
-                        (per_x && (i == 0 || i == L - 1));
-    if (xs_row) return;   // summed with the four-copy rule by the x-seam items
     int jL, jR;
     if (g < nsegs - 1) {
       jR = (g + 1) * SP;
@@ -615,13 +612,17 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
       jL = NC - 1;
       jR = 0;
     }
+    const int i = q;
+    if (i >= L) return;
+    const int r = i % CXP;
+    const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
+                        (per_x && (i == 0 || i == L - 1));
+    if (xs_row) return;   // summed with the four-copy rule by the x-seam items
     const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
     fin(i, jL, sm);
     fin(i, jR, sm);
   } else {
     const int s = blockIdx.y - nys;
-    const int j = q;
-    if (j >= NC) return;
     int iL, iR;
     if (s < nstrips - 1) {
       iR = (s + 1) * CXP;
@@ -630,6 +631,8 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
       iL = L - 1;
       iR = 0;
     }
+    const int j = q;
+    if (j >= NC) return;
     const int b = j % P;
     if (b == 0 && (j > 0 || per_y)) return;  // summed by the item of the row below
     if (b == P - 1 && (j < NC - 1 || per_y)) {
@@ -756,8 +759,8 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   if (a.assemble && pl.nk2 > 0 && !pc) {
     const int nys = pl.nsegs - 1 + (g.per_y ? 1 : 0);
     const int nxs = pl.nstrips - 1 + (g.per_x ? 1 : 0);
-    const dim3 sgrid((std::max(g.L, g.NC) + 255) / 256, nys + nxs);
-    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(256), 0, st, prm.y, g.L, g.NC, P, CX * P,
+    const dim3 sgrid((std::max(g.L, g.NC) + kSeamBlock - 1) / kSeamBlock, nys + nxs);
+    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P, CX * P,
                        pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys);
   }
   SEM_HIP(hipGetLastError());
