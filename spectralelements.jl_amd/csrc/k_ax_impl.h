@@ -68,6 +68,9 @@ struct AxP {
   const T* __restrict__ r;
   const T* __restrict__ d;
   T* __restrict__ p;
+  // FP64: x += alpha_{k-1} p_{k-1} while p_{k-1} is read (the update kernel
+  // then leaves x alone)
+  T* __restrict__ x;
   PcgDev pcg;
 };
 
@@ -204,7 +207,9 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   }
 
   const int tid = threadIdx.x;
-  T beta = T(0);
+  // FP64 PCG: the iterate is advanced here, one iteration late
+  constexpr bool kXLate = MODE == 2 && sizeof(T) == 8;
+  T beta = T(0), alpha_prev = T(0);
   bool first = false;
   if (MODE == 2) {
     if (block_done(prm.pcg.state, red)) return;
@@ -223,6 +228,7 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
     }
     first = (k == 1);
     if (!first) beta = T(t / prm.pcg.state->t[(k - 1) & 1]);
+    if (kXLate && !first) alpha_prev = T(prm.pcg.state->alpha[(k - 1) & 1]);
     if (blockIdx.x == 0 && tid == 0) prm.pcg.state->t[k & 1] = t;
   }
   double dot = 0.0;
@@ -295,6 +301,18 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
       if (first) {
 #pragma unroll
         for (int b = 0; b < P; ++b) uv[b] = zv[b];
+      } else if constexpr (kXLate) {
+        T po[P], xo[P];
+#pragma unroll
+        for (int b = 0; b < P; ++b) po[b] = prm.p[bc_ + L * b];
+#pragma unroll
+        for (int b = 0; b < P; ++b) xo[b] = prm.x[bc_ + L * b];
+#pragma unroll
+        for (int b = 0; b < P; ++b) uv[b] = zv[b] + beta * po[b];
+        if (act) {
+#pragma unroll
+          for (int b = 0; b < P; ++b) prm.x[base + L * b] = xo[b] + alpha_prev * po[b];
+        }
       } else {
 #pragma unroll
         for (int b = 0; b < P; ++b) uv[b] = zv[b] + beta * prm.p[bc_ + L * b];
@@ -714,9 +732,14 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   const dim3 grid(pl.nk1), block(CX * P);
   int mode = a.assemble ? 1 : 0;
   if (pc) {
+    if (sizeof(T) == 8 && !pc->x) {
+      set_error("FP64 PCG apply needs the iterate x");
+      return SEM_ERR_ARG;
+    }
     prm.r = static_cast<const T*>(pc->r);
     prm.d = static_cast<const T*>(pc->d);
     prm.p = static_cast<T*>(pc->p);
+    prm.x = static_cast<T*>(pc->x);
     prm.u = nullptr;
     prm.pcg = pc->dev;
     mode = 2;

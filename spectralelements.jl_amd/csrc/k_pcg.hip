@@ -78,15 +78,21 @@ __device__ inline int row_info(int i, const SeamGeom& g) {
 // the partials: a select, not a branch, so the bulk pass has no divergent
 // control flow. The max ignores NaN here; the block restores NaN from the
 // <r, z> partial, which is NaN whenever some r is.
+// For FP64 (XL) the iterate is not touched here: the next iteration's apply
+// adds alpha p while it reads p anyway, and k_pcg_xlast adds the last one.
 template <typename T, bool HAS_D>
 __device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
                                          const T* __restrict__ p, const T* __restrict__ d,
                                          long t, T wf, T alpha, double m, bool keep,
                                          double& rho, double& mx) {
-  const T xo = x[t], ro = r[t];
-  const T xn = xo + alpha * p[t];
+  constexpr bool XL = sizeof(T) == 8;
+  const T ro = r[t];
   const T rv = ro - alpha * wf;
-  x[t] = keep ? xn : xo;
+  if constexpr (!XL) {
+    const T xo = x[t];
+    const T xn = xo + alpha * p[t];
+    x[t] = keep ? xn : xo;
+  }
   r[t] = keep ? rv : ro;
   T z = rv;
   if constexpr (HAS_D) z = d[t] * rv;
@@ -132,6 +138,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
   const double pw = block_sum_w<kUpdThreads>(qs, red);   // synchronises the block
   const T alpha = T(tk / pw);
+  if (sizeof(T) == 8 && blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = (double)alpha;
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
   const int j0 = blockIdx.x * cpb;
   const int j1 = min(NC, j0 + cpb);
@@ -294,6 +301,17 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = rho != rho ? rho : mx;   // a NaN residual makes rho NaN
   }
+}
+
+// FP64 solves: x += alpha p of the last iteration (the apply that would have
+// added it found the stop test fired); nothing when no iteration ran.
+__global__ void k_pcg_xlast(double* __restrict__ x, const double* __restrict__ p,
+                            const PcgState* __restrict__ st, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int it = st->iters;
+  if (it < 1) return;
+  x[t] = x[t] + st->alpha[it & 1] * p[t];
 }
 
 // Element diagonal of nu A + k B (nu, k scalars or fields):
@@ -473,6 +491,7 @@ struct PcgRun {
         pc.r = r;
         pc.d = d;
         pc.p = p;
+        pc.x = x;
         pc.dev.part_prev = part + 2 * n3 * ((k - 1) & 1);
         pc.dev.n3 = n3;
         pc.dev.Q = Q;
@@ -488,6 +507,12 @@ struct PcgRun {
       SEM_HIP(hipStreamSynchronize(st));
       if (hs->done) {
         if (hint && hs->converged) op->last_iters = hs->iters;
+        if constexpr (sizeof(T) == 8) {
+          hipLaunchKernelGGL(k_pcg_xlast, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                             reinterpret_cast<double*>(x), reinterpret_cast<const double*>(p),
+                             state, n);
+          SEM_HIP(hipGetLastError());
+        }
         return SEM_OK;
       }
       if (k > kmax) {
