@@ -68,8 +68,8 @@ struct AxP {
   const T* __restrict__ r;
   const T* __restrict__ d;
   T* __restrict__ p;
-  // FP64: x += alpha_{k-1} p_{k-1} while p_{k-1} is read (the update kernel
-  // then leaves x alone)
+  // x += alpha_{k-1} p_{k-1} while p_{k-1} is read (the update kernel
+  // leaves x alone)
   T* __restrict__ x;
   PcgDev pcg;
 };
@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   }
 
   const int tid = threadIdx.x;
-  // FP64 PCG: the iterate is advanced here, one iteration late
-  constexpr bool kXLate = MODE == 2 && sizeof(T) == 8;
+  // PCG: the iterate is advanced here, one iteration late
+  constexpr bool kXLate = MODE == 2;
   T beta = T(0), alpha_prev = T(0);
   bool first = false;
   if (MODE == 2) {
@@ -732,8 +732,8 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   const dim3 grid(pl.nk1), block(CX * P);
   int mode = a.assemble ? 1 : 0;
   if (pc) {
-    if (sizeof(T) == 8 && !pc->x) {
-      set_error("FP64 PCG apply needs the iterate x");
+    if (!pc->x) {
+      set_error("PCG apply needs the iterate x");
       return SEM_ERR_ARG;
     }
     prm.r = static_cast<const T*>(pc->r);

@@ -72,27 +72,19 @@ __device__ inline int row_info(int i, const SeamGeom& g) {
   return v;
 }
 
-// One node of the PCG update: x += alpha p, r -= alpha wf, partials of
-// <r, z>_mult (m = the node's mult) and norm(r, Inf). With keep == false the
-// node is left as it is (its old x and r are stored back) and adds nothing to
-// the partials: a select, not a branch, so the bulk pass has no divergent
+// One node of the PCG update: r -= alpha wf, partials of <r, z>_mult (m =
+// the node's mult) and norm(r, Inf). With keep == false the node is left as
+// it is (its old r is stored back) and adds nothing to the partials: a select, not a branch, so the bulk pass has no divergent
 // control flow. The max ignores NaN here; the block restores NaN from the
 // <r, z> partial, which is NaN whenever some r is.
-// For FP64 (XL) the iterate is not touched here: the next iteration's apply
-// adds alpha p while it reads p anyway, and k_pcg_xlast adds the last one.
+// The iterate is not touched here: the next iteration's apply adds alpha p
+// while it reads p anyway, and k_pcg_xlast adds the last one.
 template <typename T, bool HAS_D>
-__device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
-                                         const T* __restrict__ p, const T* __restrict__ d,
-                                         long t, T wf, T alpha, double m, bool keep,
-                                         double& rho, double& mx) {
-  constexpr bool XL = sizeof(T) == 8;
+__device__ __forceinline__ void upd_node(T* __restrict__ r, const T* __restrict__ d, long t,
+                                         T wf, T alpha, double m, bool keep, double& rho,
+                                         double& mx) {
   const T ro = r[t];
   const T rv = ro - alpha * wf;
-  if constexpr (!XL) {
-    const T xo = x[t];
-    const T xn = xo + alpha * p[t];
-    x[t] = keep ? xn : xo;
-  }
   r[t] = keep ? rv : ro;
   T z = rv;
   if constexpr (HAS_D) z = d[t] * rv;
@@ -100,7 +92,7 @@ __device__ __forceinline__ void upd_node(T* __restrict__ x, T* __restrict__ r,
   mx = fmax(mx, keep ? fabs((double)rv) : 0.0);
 }
 
-// x += alpha p; r -= alpha w (w's seam copies finished here with the k_seams
+// r -= alpha w (w's seam copies finished here with the k_seams
 // rules: same operands, same order, so every copy gets identical bits);
 // partials of <r, z>_mult and norm(r, Inf) for the next iteration. A block
 // walks whole columns j (its share of NC). Bulk pass: each thread takes rows
@@ -117,8 +109,7 @@ constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP
 // count (scalar loop control, no per-lane exit masks).
 template <typename T, bool HAS_D, bool FULL, int VEC = 1>
 __global__ void __launch_bounds__(kUpdThreads)
-k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-             const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
+k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
              int nQ, PcgState* __restrict__ st, int k, double* __restrict__ part, int n3,
              SeamGeom sg, long n) {
   const int P = sg.P, NC = sg.NC, per_y = sg.per_y;
@@ -138,7 +129,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
   const double pw = block_sum_w<kUpdThreads>(qs, red);   // synchronises the block
   const T alpha = T(tk / pw);
-  if (sizeof(T) == 8 && blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = (double)alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = (double)alpha;
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
   const int j0 = blockIdx.x * cpb;
   const int j1 = min(NC, j0 + cpb);
@@ -184,21 +175,14 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
         } else {
           wf = *reinterpret_cast<const T2*>(w + t);
         }
-        const T2 xo = *reinterpret_cast<const T2*>(x + t);
         const T2 ro = *reinterpret_cast<const T2*>(r + t);
-        const T2 pv = *reinterpret_cast<const T2*>(p + t);
         const bool k0 = !(info.x & 6), k1 = !(info.y & 6);
-        T2 xn, rn;
-        xn.x = xo.x + alpha * pv.x;
-        xn.y = xo.y + alpha * pv.y;
+        T2 rn;
         rn.x = ro.x - alpha * wf.x;
         rn.y = ro.y - alpha * wf.y;
-        T2 xs, rs;
-        xs.x = k0 ? xn.x : xo.x;
-        xs.y = k1 ? xn.y : xo.y;
+        T2 rs;
         rs.x = k0 ? rn.x : ro.x;
         rs.y = k1 ? rn.y : ro.y;
-        *reinterpret_cast<T2*>(x + t) = xs;
         *reinterpret_cast<T2*>(r + t) = rs;
         T2 z = rn;
         if constexpr (HAS_D) {
@@ -220,7 +204,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
           const int info = rinfo[i];
           const T mv = ((info & 8) || masky) ? T(0) : T(1);
           const T wf = (w[i + cL] + w[i + cR]) * mv;
-          upd_node<T, HAS_D>(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
+          upd_node<T, HAS_D>(r, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
                              !(info & 6), rho, mx);
         }
       } else {
@@ -229,7 +213,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
           const int i = i0 + (int)threadIdx.x;
           const int info = rinfo[i];
           const long t = i + cj;
-          upd_node<T, HAS_D>(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
+          upd_node<T, HAS_D>(r, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
                              !(info & 6), rho, mx);
         }
       }
@@ -239,14 +223,14 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
         const int info = table ? rinfo[i] : row_info(i, sg);
         const T mv = ((info & 8) || masky) ? T(0) : T(1);
         const T wf = (w[i + cL] + w[i + cR]) * mv;
-        upd_node<T, HAS_D>(x, r, p, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
+        upd_node<T, HAS_D>(r, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
                            !(info & 6), rho, mx);
       }
     } else {
       for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
         const int info = table ? rinfo[i] : row_info(i, sg);
         const long t = i + cj;
-        upd_node<T, HAS_D>(x, r, p, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
+        upd_node<T, HAS_D>(r, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
                            !(info & 6), rho, mx);
       }
     }
@@ -292,7 +276,7 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     } else {
       sv = w[iL + L * j] + w[iR + L * j];
     }
-    upd_node<T, HAS_D>(x, r, p, d, i + L * j, sv * mv, alpha,
+    upd_node<T, HAS_D>(r, d, i + L * j, sv * mv, alpha,
                        (info & 1 ? 0.5 : 1.0) * (yhalf ? 0.5 : 1.0), true, rho, mx);
   }
   rho = block_sum_w<kUpdThreads>(rho, red);
@@ -303,15 +287,23 @@ k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   }
 }
 
-// FP64 solves: x += alpha p of the last iteration (the apply that would have
-// added it found the stop test fired); nothing when no iteration ran.
-__global__ void k_pcg_xlast(double* __restrict__ x, const double* __restrict__ p,
+// x += alpha p of the last iteration (the apply that would have added it
+// found the stop test fired); nothing when no iteration ran.
+template <typename T>
+__global__ void k_pcg_xlast(T* __restrict__ x, const T* __restrict__ p,
                             const PcgState* __restrict__ st, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int it = st->iters;
   if (it < 1) return;
-  x[t] = x[t] + st->alpha[it & 1] * p[t];
+  x[t] = x[t] + T(st->alpha[it & 1]) * p[t];
+}
+
+// After k_pcg_xlast: that alpha p is in x, so a solve continued from the
+// next iteration (kept search direction) must not add it again.
+__global__ void k_pcg_alpha_clear(PcgState* st) {
+  const int it = st->iters;
+  if (it >= 1) st->alpha[it & 1] = 0.0;
 }
 
 // Element diagonal of nu A + k B (nu, k scalars or fields):
@@ -449,7 +441,7 @@ struct PcgRun {
     return SEM_OK;
   }
 
-  void launch_update(hipStream_t st, T* x, int k) {
+  void launch_update(hipStream_t st, int k) {
     const size_t lds = g.L <= kRowTable ? sizeof(int) * g.L : 0;
     const bool full = g.L % kUpdThreads == 0 && g.L <= kRowTable;
     // FP32 with rows in pairs: L a multiple of 2 kUpdThreads (8-byte aligned
@@ -457,7 +449,7 @@ struct PcgRun {
     const bool pairs = sizeof(T) == 4 && g.L % (2 * kUpdThreads) == 0 && g.L <= kRowTable;
     double* pk = part + 2 * n3 * (k & 1);
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, x, r, p, w, d, Q, nQ, state,
+      hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, r, w, d, Q, nQ, state,
                          k, pk, n3, sg, n);
     };
     if constexpr (sizeof(T) == 4) {
@@ -500,19 +492,17 @@ struct PcgRun {
         pc.dev.maxiter = maxiter;
         pc.dev.k = k;
         SEM_TRY(launch_apply_pcg<T>(st, g, aa, pc));
-        launch_update(st, x, k);
+        launch_update(st, k);
       }
       SEM_HIP(hipGetLastError());
       SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
       SEM_HIP(hipStreamSynchronize(st));
       if (hs->done) {
         if (hint && hs->converged) op->last_iters = hs->iters;
-        if constexpr (sizeof(T) == 8) {
-          hipLaunchKernelGGL(k_pcg_xlast, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                             reinterpret_cast<double*>(x), reinterpret_cast<const double*>(p),
-                             state, n);
-          SEM_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL((k_pcg_xlast<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           st, x, p, state, n);
+        hipLaunchKernelGGL(k_pcg_alpha_clear, dim3(1), dim3(1), 0, st, state);
+        SEM_HIP(hipGetLastError());
         return SEM_OK;
       }
       if (k > kmax) {

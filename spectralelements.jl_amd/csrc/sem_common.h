@@ -154,8 +154,8 @@ struct PcgState {
   int pad;
   double res;
   double t[2];
-  // alpha of iteration k at [k & 1] (FP64 solves: x += alpha p is applied by
-  // the next iteration's apply, and by a finishing pass after the last one)
+  // alpha of iteration k at [k & 1]: x += alpha p is applied by the next
+  // iteration's apply, and by a finishing pass after the last one
   double alpha[2];
 };
 
@@ -180,7 +180,7 @@ struct PcgLaunch {
   const void* r;  // recursive residual
   const void* d;  // diagonal preconditioner (NULL = identity)
   void* p;        // search direction, updated in place to z + beta p
-  void* x;        // FP64: the iterate, advanced by the previous alpha p
+  void* x;        // the iterate, advanced by the previous alpha p
   PcgDev dev;
 };
 template <typename T>
