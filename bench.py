@@ -346,6 +346,8 @@ def main():
         res["pcg_fp64"] = bench_pcg(se, m, sets)
         it = res["pcg_fp64"]
         res["pcg_fp64"]["achieved_GBps_104B"] = 104 * nl / (it["ms_per_iter"] * 1e-3) / 1e9
+        # bytes the two launches move per node: apply 88 B, update 32 B
+        res["pcg_fp64"]["moved_GBps_120B"] = 120 * nl / (it["ms_per_iter"] * 1e-3) / 1e9
     del sets
     torch.cuda.empty_cache()
     if not args.no_mixed:

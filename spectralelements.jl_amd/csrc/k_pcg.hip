@@ -3,15 +3,17 @@
 // Per iteration k (all scalars stay on the device):
 //   apply kernel (k_ax MODE 2): checks the stop test on the partials of the
 //       previous update (norm(r, Inf) <= tol, or k-1 == maxiter), forms
-//       t = <r, h>, beta = t / t_prev, u = h + beta u (h = d .* r or r),
+//       t = <r, h>, beta = t / t_prev, u = h + beta u (h = d .* r or r)
+//       and, while the old u is in registers, x += alpha_{k-1} u_old;
 //       w = M .* QQ'(A u) except at workgroup seams, and the partials of
 //       <u, w>_mult, which equal sum(u .* H) over the unassembled element
 //       outputs H because u is continuous and masked;
 //   update kernel: finishes the seam copies of w on the fly (same order as
-//       the seam kernel of the plain apply), alpha = t / <u, w>,
-//       x += alpha u, r -= alpha w, and the partials of <r, h> and
-//       norm(r, Inf) for the next iteration.
-// Two launches per iteration.
+//       the seam kernel of the plain apply), alpha = t / <u, w> (kept in the
+//       state for the next apply), r -= alpha w, and the partials of <r, h>
+//       and norm(r, Inf) for the next iteration.
+// Two launches per iteration; after the stop test one pass adds the last
+// alpha u to x. x follows the same sequence as x = x + a * u in pcg.jl:53.
 // Partials are summed in a fixed order by every block that needs them, so
 // every block sees bit-identical scalars and the solve is deterministic.
 // The host launches iterations in growing batches and reads the done flag
@@ -235,7 +237,7 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
       }
     }
   }
-  __syncthreads();   // the bulk stored the seam rows' old x, r; the seam pass follows
+  __syncthreads();   // the bulk stored the seam rows' old r; the seam pass follows
   // strip-seam rows: s*CXP - 1 and s*CXP for s = 1 .. nstrips-1, then the
   // periodic wrap rows 0 and L-1
   const int nsr = 2 * (sg.nstrips - 1);
