@@ -281,8 +281,7 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
     upd_node<T, HAS_D>(r, d, i + L * j, sv * mv, alpha,
                        (info & 1 ? 0.5 : 1.0) * (yhalf ? 0.5 : 1.0), true, rho, mx);
   }
-  rho = block_sum_w<kUpdThreads>(rho, red);
-  mx = block_max_w<kUpdThreads>(mx, red);
+  block_sum_max_to0<kUpdThreads>(rho, mx, red);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = rho != rho ? rho : mx;   // a NaN residual makes rho NaN

@@ -97,6 +97,29 @@ __device__ inline double block_max_w(double v, double* red) {
   return r;
 }
 
+// Block sum of s and block max of m in one pass, valid in thread 0 only (the
+// same reduction trees as block_sum_w / block_max_w, one barrier instead of
+// six). red must hold 2 NT / 64 values and be free.
+template <int NT>
+__device__ inline void block_sum_max_to0(double& s, double& m, double* red) {
+  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  s = wave_sum(s);
+  m = wave_max(m);
+  if (lane == 0) {
+    red[wv] = s;
+    red[NW + wv] = m;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    double ts = lane < NW ? red[lane] : 0.0;
+    double tm = lane < NW ? red[NW + lane] : 0.0;
+    s = wave_sum(ts);
+    m = wave_max(tm);
+  }
+}
+
 // Sum of part[0..n) and max of part[n..2n), identical in every block.
 __device__ inline void reduce_partials(const double* part, int n, double* red, int nthreads,
                                        double* sum, double* mx) {
