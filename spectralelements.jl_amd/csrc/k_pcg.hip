@@ -125,11 +125,15 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
   double qs = 0.0;
   for (int q = threadIdx.x; q < nQ; q += kUpdThreads) qs += Q[q];
   const double tk = st->t[k & 1];
-  if (block_done(st, red)) return;
+  // done is written only by the apply launches, so it cannot change while
+  // this kernel runs: every thread reads the same value, no broadcast needed
+  if (st->done) return;
   const bool table = FULL || Lr <= kRowTable;   // uniform
   if (table)
     for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
-  const double pw = block_sum_w<kUpdThreads>(qs, red);   // synchronises the block
+  // synchronises the block; red is next written by the final reduction,
+  // after the barrier that precedes the seam pass
+  const double pw = block_sum_w_nb<kUpdThreads>(qs, red);
   const T alpha = T(tk / pw);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = (double)alpha;
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;

@@ -79,6 +79,24 @@ __device__ inline double block_sum_w(double v, double* red) {
   return r;
 }
 
+// block_sum_w without the trailing barrier: red[0] holds the result until
+// the caller's next barrier, so red must not be written before one.
+template <int NT>
+__device__ inline double block_sum_w_nb(double v, double* red) {
+  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  v = wave_sum(v);
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    double t = lane < NT / 64 ? red[lane] : 0.0;
+    t = wave_sum(t);
+    if (lane == 0) red[0] = t;
+  }
+  __syncthreads();
+  return red[0];
+}
+
 template <int NT>
 __device__ inline double block_max_w(double v, double* red) {
   static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
