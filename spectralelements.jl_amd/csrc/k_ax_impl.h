@@ -181,6 +181,9 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   // x-pair exchange buffer (R3 reads s1, which is s0 on the in-place path)
   __shared__ T xbuf[(MF || kInPlace) ? 2 * NR : 1];
   __shared__ double red[NR];
+  // PCG mode: the max tree of the partials and the stop flag
+  __shared__ double red2[MODE == 2 ? NR : 1];
+  __shared__ int flag;
   // MFMA A-operands: D and D' (Dr = Ds); hi and lo parts (MF 1..4) or the
   // four K-blocks (MF 5, 6)
   using Frag = typename MfFrag<(MF >= 1 && MF <= 4) ? MF : 1>::type;
@@ -212,9 +215,11 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   T beta = T(0), alpha_prev = T(0);
   bool first = false;
   if (MODE == 2) {
-    if (block_done(prm.pcg.state, red)) return;
     double t, res;
-    reduce_partials(prm.pcg.part_prev, prm.pcg.n3, red, NR, &t, &res);
+    bool stopped;
+    reduce_partials_flag(prm.pcg.part_prev, prm.pcg.n3, red, red2, &flag, NR, prm.pcg.state, &t,
+                         &res, &stopped);
+    if (stopped) return;
     const int k = prm.pcg.k;
     const bool done = !(res > prm.pcg.tol) || (k - 1 == prm.pcg.maxiter);
     if (done) {

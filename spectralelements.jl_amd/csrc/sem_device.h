@@ -138,32 +138,38 @@ __device__ inline void block_sum_max_to0(double& s, double& m, double* red) {
   }
 }
 
-// Sum of part[0..n) and max of part[n..2n), identical in every block.
-__device__ inline void reduce_partials(const double* part, int n, double* red, int nthreads,
-                                       double* sum, double* mx) {
+// Sum of part[0..n) and max of part[n..2n), identical in every block, fused
+// with the stop-flag broadcast: the sum and the max climb the same LDS trees
+// as block_sum / block_max side by side, and state->done, read by thread 0,
+// is published by the first barrier (one tree's barriers instead of two
+// trees' plus two).
+__device__ inline void reduce_partials_flag(const double* part, int n, double* red, double* red2,
+                                            int* flag, int nthreads, const PcgState* st,
+                                            double* sum, double* mx, bool* stopped) {
+  const int tid = threadIdx.x;
   double s = 0.0, m = 0.0;
-  for (int q = threadIdx.x; q < n; q += nthreads) {
+  for (int q = tid; q < n; q += nthreads) {
     s += part[q];
     m = nanmax(m, part[n + q]);
   }
-  *sum = block_sum(s, red, nthreads);
-  *mx = block_max(m, red, nthreads);
-}
-
-__device__ inline double reduce_sum_array(const double* part, int n, double* red, int nthreads) {
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += nthreads) s += part[q];
-  return block_sum(s, red, nthreads);
-}
-
-// Reads state->done once per block and broadcasts it, so the whole block
-// takes the same branch.
-__device__ inline bool block_done(const PcgState* st, double* red) {
-  if (threadIdx.x == 0) red[0] = (double)st->done;
+  if (tid == 0) *flag = st->done;
+  red[tid] = s;
+  red2[tid] = m;
   __syncthreads();
-  const bool d = red[0] != 0.0;
+  int c = nthreads;
+  while (c > 1) {
+    const int h = (c + 1) / 2;
+    if (tid < c / 2) {
+      red[tid] = red[tid] + red[tid + h];
+      red2[tid] = nanmax(red2[tid], red2[tid + h]);
+    }
+    __syncthreads();
+    c = h;
+  }
+  *sum = red[0];
+  *mx = red2[0];
+  *stopped = *flag != 0;
   __syncthreads();
-  return d;
 }
 
 // 1/(multiplicity along one direction) of local row i (P points per element)
