@@ -59,10 +59,14 @@ __device__ inline double block_max(double v, double* red, int nthreads) {
 }
 
 // The same reductions for blocks of NT threads, NT a multiple of 64: lane
-// shuffles within each wavefront, then one LDS step across the waves (three
-// barriers instead of log2(NT)). Fixed order, bit-identical across blocks.
+// shuffles within each wavefront, then one LDS step across the waves. Fixed
+// order, bit-identical across blocks.
+//
+// Block sum returned to every thread with two barriers: red[0] holds the
+// result until the caller's next barrier, so red must not be written before
+// one.
 template <int NT>
-__device__ inline double block_sum_w(double v, double* red) {
+__device__ inline double block_sum_w_nb(double v, double* red) {
   static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   v = wave_sum(v);
@@ -74,50 +78,11 @@ __device__ inline double block_sum_w(double v, double* red) {
     if (lane == 0) red[0] = t;   // wave 0 has read red[0..NT/64) already
   }
   __syncthreads();
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// block_sum_w without the trailing barrier: red[0] holds the result until
-// the caller's next barrier, so red must not be written before one.
-template <int NT>
-__device__ inline double block_sum_w_nb(double v, double* red) {
-  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  v = wave_sum(v);
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  if (wv == 0) {
-    double t = lane < NT / 64 ? red[lane] : 0.0;
-    t = wave_sum(t);
-    if (lane == 0) red[0] = t;
-  }
-  __syncthreads();
   return red[0];
 }
 
-template <int NT>
-__device__ inline double block_max_w(double v, double* red) {
-  static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  v = wave_max(v);
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  if (wv == 0) {
-    double t = lane < NT / 64 ? red[lane] : 0.0;
-    t = wave_max(t);
-    if (lane == 0) red[0] = t;
-  }
-  __syncthreads();
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// Block sum of s and block max of m in one pass, valid in thread 0 only (the
-// same reduction trees as block_sum_w / block_max_w, one barrier instead of
-// six). red must hold 2 NT / 64 values and be free.
+// Block sum of s and block max of m in one pass, valid in thread 0 only
+// (one barrier). red must hold 2 NT / 64 values and be free.
 template <int NT>
 __device__ inline void block_sum_max_to0(double& s, double& m, double* red) {
   static_assert(NT % 64 == 0 && NT <= 4096, "whole wavefronts");
