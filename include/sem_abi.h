@@ -244,6 +244,51 @@ int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op
                   const double* d64, const double* b, double* x, double tol,
                   double inner_tol, int max_outer, int max_inner, sem_pcg_info* info);
 
+/* --------------------------------------------- multigrid preconditioner */
+/* A stronger opM for pcg than the identity / Jacobi of diffusion.jl:47-49
+ * (the reference's own next step is the commented element fast-diagonalisation
+ * sketch of lapl.jl:105-118 and p2d_explicit.jl:108-141). A symmetric V-cycle:
+ * p-multigrid on the element grid (N -> N/2 -> ... -> 1), then h-multigrid on
+ * bilinear elements (2 x 2 elements -> 1 while both element counts are even
+ * and at least 4), each level the fused operator mask(QQ'(nu A + k B))
+ * rediscretised on a mesh whose nodes sample the fine geometry, Chebyshev-
+ * Jacobi smoothing of degree `degree` on [lmax/ratio, lmax] (lmax = safety x
+ * the Rayleigh quotient after npow power iterations from a hashed start
+ * vector), Chebyshev of degree `degree_coarse` on the coarsest level.
+ * Prolongation interpolates per element; restriction is its transpose
+ * (QQ'_c(J' (mult .* r) J), masked). Needs an FP64, assembled operator with
+ * scalar nu and k (no field or factor overrides). */
+typedef struct sem_mg_params {
+  int degree;         /* default 3 */
+  double ratio;       /* default 20 */
+  int degree_coarse;  /* default 8 */
+  int npow;           /* default 20 */
+  double safety;      /* default 1.1 */
+  int max_levels;     /* default 32 */
+} sem_mg_params;
+
+typedef struct sem_mg sem_mg;
+
+/* fills the defaults listed above */
+int sem_mg_default_params(sem_mg_params* params);
+/* builds the hierarchy for op (params NULL: defaults); the operator and its
+ * mesh must outlive the object */
+int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* params,
+                  sem_mg** out);
+int sem_mg_destroy(sem_mg* mg);
+int sem_mg_num_levels(const sem_mg* mg, int* nlevels);
+/* level 0 is the operator's own mesh; p = points per direction */
+int sem_mg_level_info(const sem_mg* mg, int level, int* p, int* Ex, int* Ey, double* lmax);
+/* replaces a level's lmax (the smoothing interval follows) */
+int sem_mg_set_lmax(sem_mg* mg, int level, double lmax);
+/* z = V(r): one V-cycle applied to an assembled, masked residual r (FP64) */
+int sem_mg_vcycle(sem_ctx* ctx, sem_mg* mg, const double* r, double* z);
+/* pcg(b, opA; opM = V-cycle, mult, tol, maxiter), pcg.jl:16-60: x0 = 0,
+ * stop when norm(r, Inf) <= tol (absolute), mult-weighted inner products,
+ * SEM_WARN_MAXITER at k == maxiter. op must be the operator mg was built for. */
+int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b, double* x,
+               double tol, int maxiter, sem_pcg_info* info);
+
 /* ----------------------------------------------------- vector utilities */
 /* sum(a .* b .* mult) with the mesh's mult (pcg.jl:45); result to host */
 int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* b,

@@ -1,0 +1,656 @@
+// Multigrid preconditioner for the fused SEM operator and PCG with the
+// V-cycle as opM (pcg.jl:16-60 with a stronger preconditioner than the
+// identity / Jacobi of diffusion.jl:47-49; the reference's own next step is
+// the commented element fast-diagonalisation sketch of lapl.jl:105-118).
+//
+// Hierarchy: p-multigrid on the element grid (N -> N/2 -> ... -> 1), then
+// h-multigrid on bilinear elements (2 x 2 elements -> 1). Every level is a
+// library mesh + fused operator (k_ax + k_seams), rediscretised with the same
+// scalar nu, k and Dirichlet bits on nodes that sample the fine geometry.
+// Smoother: Chebyshev-Jacobi (Saad, Iterative Methods for Sparse Linear
+// Systems, Alg. 12.1) with host-side coefficients; transfers: one thread per
+// output node, the element's interpolation rows from a small table.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "sem_device.h"
+#include "sem_l0.h"
+
+namespace sem {
+namespace {
+
+inline int nblk(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
+
+// Chebyshev start: d = dinv .* r / theta; x = d (x0 = 0, and r = b copied
+// from b) or x += d.
+__global__ void k_cheb_start(const double* __restrict__ b, double* __restrict__ r,
+                             const double* __restrict__ dinv, double* __restrict__ d,
+                             double* __restrict__ x, double theta, int from_zero, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double rv;
+  if (from_zero) {
+    rv = b[t];
+    r[t] = rv;
+  } else {
+    rv = r[t];
+  }
+  const double dv = dinv[t] * rv / theta;
+  d[t] = dv;
+  x[t] = from_zero ? dv : x[t] + dv;
+}
+
+// One Chebyshev step after w = A d: r -= w; d = c1 d + c2 dinv .* r; x += d
+__global__ void k_cheb_step(const double* __restrict__ w, double* __restrict__ r,
+                            const double* __restrict__ dinv, double* __restrict__ d,
+                            double* __restrict__ x, double c1, double c2, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double rv = r[t] - w[t];
+  r[t] = rv;
+  const double dv = c1 * d[t] + c2 * (dinv[t] * rv);
+  d[t] = dv;
+  x[t] = x[t] + dv;
+}
+
+__global__ void k_sub(double* __restrict__ r, const double* __restrict__ w, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) r[t] = r[t] - w[t];
+}
+
+// Transfer geometry: fine element ef = (ec << h) + sub; J holds nsub
+// row-major (pf x pc) blocks for x, then for y.
+struct Xfer {
+  int pf, pc, hx, hy;
+  int Lf, NCf, Lc, NCc;
+  int per_x, per_y;
+};
+
+// e = P xc (per fine element: Jx xc Jy'), x += e
+__global__ void k_prolong(const double* __restrict__ xc, const double* __restrict__ J,
+                          double* __restrict__ e, double* __restrict__ x, Xfer g, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.Lf), j = (int)(t / g.Lf);
+  const int exf = i / g.pf, a = i - exf * g.pf;
+  const int eyf = j / g.pf, b = j - eyf * g.pf;
+  const int sx = exf & g.hx, sy = eyf & g.hy;
+  const int exc = exf >> g.hx, eyc = eyf >> g.hy;
+  const int blk = g.pf * g.pc;
+  const double* jx = J + sx * blk + a * g.pc;
+  const double* jy = J + (g.hx ? 2 : 1) * blk + sy * blk + b * g.pc;
+  const long c0 = (long)exc * g.pc + (long)g.Lc * ((long)eyc * g.pc);
+  double v = 0.0;
+  for (int bb = 0; bb < g.pc; ++bb) {
+    const double wy = jy[bb];
+    double s = 0.0;
+    for (int aa = 0; aa < g.pc; ++aa) s += jx[aa] * xc[c0 + aa + (long)g.Lc * bb];
+    v += wy * s;
+  }
+  e[t] = v;
+  x[t] = x[t] + v;
+}
+
+// rc (element-local) = Jx' (mult_f .* rf) Jy summed over the fine nodes of
+// the coarse element; assembled and masked afterwards by the coarse QQ'
+__global__ void k_restrict(const double* __restrict__ rf, const double* __restrict__ J,
+                           double* __restrict__ rc, Xfer g, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.Lc), j = (int)(t / g.Lc);
+  const int exc = i / g.pc, a = i - exc * g.pc;
+  const int eyc = j / g.pc, b = j - eyc * g.pc;
+  const int blk = g.pf * g.pc;
+  const int nsx = g.hx ? 2 : 1, nsy = g.hy ? 2 : 1;
+  const double* Jy0 = J + nsx * blk;
+  double v = 0.0;
+  for (int sy = 0; sy < nsy; ++sy) {
+    const int eyf = (eyc << g.hy) + sy;
+    for (int bf = 0; bf < g.pf; ++bf) {
+      const int jf = eyf * g.pf + bf;
+      const double wy = Jy0[sy * blk + bf * g.pc + b] * mult1(jf, g.pf, g.NCf, g.per_y);
+      double s = 0.0;
+      for (int sx = 0; sx < nsx; ++sx) {
+        const int exf = (exc << g.hx) + sx;
+        for (int af = 0; af < g.pf; ++af) {
+          const int ifn = exf * g.pf + af;
+          s += J[sx * blk + af * g.pc + a] * mult1(ifn, g.pf, g.Lf, g.per_x) *
+               rf[ifn + (long)g.Lf * jf];
+        }
+      }
+      v += wy * s;
+    }
+  }
+  rc[t] = v;
+}
+
+// coarse coordinates: the fine field interpolated at the coarse nodes; Jc is
+// (pc x pf) row-major, node a of a coarse element lies in fine sub-element
+// (subbits >> a) & 1 (h-merge only)
+__global__ void k_coarse_coords(const double* __restrict__ xf, const double* __restrict__ Jc,
+                                int subbits, double* __restrict__ xc, Xfer g, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.Lc), j = (int)(t / g.Lc);
+  const int exc = i / g.pc, a = i - exc * g.pc;
+  const int eyc = j / g.pc, b = j - eyc * g.pc;
+  const int exf = (exc << g.hx) + (g.hx ? (subbits >> a) & 1 : 0);
+  const int eyf = (eyc << g.hy) + (g.hy ? (subbits >> b) & 1 : 0);
+  const long f0 = (long)exf * g.pf + (long)g.Lf * ((long)eyf * g.pf);
+  double v = 0.0;
+  for (int bf = 0; bf < g.pf; ++bf) {
+    double s = 0.0;
+    for (int af = 0; af < g.pf; ++af) s += Jc[a * g.pf + af] * xf[f0 + af + (long)g.Lf * bf];
+    v += Jc[b * g.pf + bf] * s;
+  }
+  xc[t] = v;
+}
+
+// hashed start vector of the power iteration: M .* (h(g) / 2^32 - 1/2),
+// h(g) = (g * 2654435761 + 12345) mod 2^32 of the global node id
+__global__ void k_hash_start(const int32_t* __restrict__ l2g, double* __restrict__ v, int L,
+                             int NC, int bc, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t h = (uint32_t)l2g[t] * 2654435761u + 12345u;
+  const int i = (int)(t % L), j = (int)(t / L);
+  v[t] = mask_on(i, j, L, NC, bc) ? (double)h / 4294967296.0 - 0.5 : 0.0;
+}
+
+__global__ void k_mul2(const double* __restrict__ a, const double* __restrict__ b,
+                       double* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = a[t] * b[t];
+}
+
+__global__ void k_div_scalar(const double* __restrict__ a, const double* __restrict__ s,
+                             double* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = a[t] / s[0];
+}
+
+// v .* D (D = 1 ./ dinv where dinv != 0, else 0): the diagonal weight of the
+// Rayleigh quotient's denominator
+__global__ void k_times_diag(const double* __restrict__ v, const double* __restrict__ dinv,
+                             double* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const double di = dinv[t];
+    y[t] = di != 0.0 ? v[t] * (1.0 / di) : 0.0;
+  }
+}
+
+// x += alpha p, r -= alpha w
+__global__ void k_xr(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                     const double* __restrict__ w, double alpha, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    x[t] = x[t] + alpha * p[t];
+    r[t] = r[t] - alpha * w[t];
+  }
+}
+
+// Interpolation matrix rows with exact unit rows where an output point equals
+// an input point (so copies of a shared node interpolate to identical bits).
+std::vector<double> interp_rows(int no, const double* xo, int ni, const double* xi) {
+  std::vector<double> Jc((size_t)no * ni), J((size_t)no * ni);
+  interpmat(no, xo, ni, xi, Jc.data());   // column-major
+  for (int i = 0; i < no; ++i)
+    for (int j = 0; j < ni; ++j) J[(size_t)i * ni + j] = Jc[i + (size_t)no * j];
+  for (int i = 0; i < no; ++i)
+    for (int j = 0; j < ni; ++j)
+      if (xo[i] == xi[j]) {
+        for (int q = 0; q < ni; ++q) J[(size_t)i * ni + q] = q == j ? 1.0 : 0.0;
+        break;
+      }
+  return J;
+}
+
+}  // namespace
+}  // namespace sem
+
+using namespace sem;
+
+struct sem_mg {
+  struct Level {
+    sem_mesh* mesh = nullptr;
+    bool own_mesh = false;
+    const sem_operator* op = nullptr;
+    bool own_op = false;
+    double* dinv = nullptr;
+    double lmax = 0.0;
+    int h = 0;            // the next level merges this level's elements 2 x 2
+    double* J = nullptr;  // transfer rows from the next level onto this one
+    Xfer xf{};
+    double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *w = nullptr, *e = nullptr;
+  };
+  sem_mg_params prm{};
+  int device = 0;
+  const sem_operator* op = nullptr;
+  std::vector<Level> lv;
+  // PCG vectors on the fine level, reduction partials and the scalar slot
+  double *pr = nullptr, *pp = nullptr, *pw = nullptr, *pz = nullptr, *part = nullptr;
+};
+
+namespace {
+
+constexpr int kParts = 256;
+
+void mg_free(sem_mg* mg) {
+  if (!mg) return;
+  (void)hipDeviceSynchronize();
+  for (auto& L : mg->lv) {
+    double* bufs[] = {L.dinv, L.J, L.r, L.d, L.w, L.e};
+    for (double* p : bufs)
+      if (p) (void)hipFree(p);
+    if (L.own_mesh) {
+      if (L.x) (void)hipFree(L.x);
+      if (L.b) (void)hipFree(L.b);
+    }
+    if (L.own_op) sem_operator_destroy(const_cast<sem_operator*>(L.op));
+    if (L.own_mesh) sem_mesh_destroy(L.mesh);
+  }
+  double* bufs[] = {mg->pr, mg->pp, mg->pw, mg->pz, mg->part};
+  for (double* p : bufs)
+    if (p) (void)hipFree(p);
+  delete mg;
+}
+
+int dalloc(double** p, long n) {
+  SEM_HIP(hipMalloc(p, sizeof(double) * std::max<long>(n, 1)));
+  return SEM_OK;
+}
+
+// host value of a device reduction result
+int fetch(sem_ctx* ctx, const double* dev, double* out) {
+  SEM_TRY(ensure_pinned(ctx, 128));
+  double* h = reinterpret_cast<double*>(static_cast<char*>(ctx->pinned) + 64);
+  SEM_HIP(hipMemcpyAsync(h, dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SEM_HIP(hipStreamSynchronize(ctx->stream));
+  *out = *h;
+  return SEM_OK;
+}
+
+int wdot_host(sem_ctx* ctx, sem_mg* mg, const Layout& g, const double* a, const double* b,
+              double* out) {
+  SEM_TRY(launch_wdot(ctx->stream, g, a, b, mg->part, kParts, mg->part + kParts));
+  return fetch(ctx, mg->part + kParts, out);
+}
+
+int norm_host(sem_ctx* ctx, sem_mg* mg, long n, const double* a, double* out) {
+  SEM_TRY(launch_norm_inf(ctx->stream, n, a, mg->part, kParts, mg->part + kParts));
+  return fetch(ctx, mg->part + kParts, out);
+}
+
+int apply(sem_ctx* ctx, const sem_operator* op, const double* u, double* y) {
+  return sem_operator_apply(ctx, op, u, y);
+}
+
+// Chebyshev iteration of degree deg for D^-1 A on [lmax / ratio, lmax]; from
+// zero (x = 0, r = b) or from (x, r). With want_r the residual of the final
+// x is formed in r (one more apply).
+int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& L, const double* b, int from_zero, int deg,
+          bool want_r) {
+  hipStream_t st = ctx->stream;
+  const long n = (long)L.mesh->nl;
+  const double lmax = L.lmax, lmin = L.lmax / mg->prm.ratio;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  hipLaunchKernelGGL(k_cheb_start, dim3(nblk(n)), dim3(256), 0, st, b, L.r, L.dinv, L.d, L.x,
+                     theta, from_zero, n);
+  SEM_HIP(hipGetLastError());
+  for (int i = 1; i < deg; ++i) {
+    SEM_TRY(apply(ctx, L.op, L.d, L.w));
+    const double rho_n = 1.0 / (2.0 * sigma - rho);
+    hipLaunchKernelGGL(k_cheb_step, dim3(nblk(n)), dim3(256), 0, st, L.w, L.r, L.dinv, L.d, L.x,
+                       rho_n * rho, 2.0 * rho_n / delta, n);
+    SEM_HIP(hipGetLastError());
+    rho = rho_n;
+  }
+  if (want_r) {
+    SEM_TRY(apply(ctx, L.op, L.d, L.w));
+    hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
+    SEM_HIP(hipGetLastError());
+  }
+  return SEM_OK;
+}
+
+// V-cycle on level l: L.x = V(L.b)
+int vcycle(sem_ctx* ctx, sem_mg* mg, int l) {
+  hipStream_t st = ctx->stream;
+  sem_mg::Level& L = mg->lv[l];
+  const int last = (int)mg->lv.size() - 1;
+  if (l == last) return cheby(ctx, mg, L, L.b, 1, mg->prm.degree_coarse, false);
+  SEM_TRY(cheby(ctx, mg, L, L.b, 1, mg->prm.degree, true));
+  sem_mg::Level& C = mg->lv[l + 1];
+  const long nc = (long)C.mesh->nl;
+  hipLaunchKernelGGL(k_restrict, dim3(nblk(nc)), dim3(256), 0, st, L.r, L.J, C.b, L.xf, nc);
+  SEM_HIP(hipGetLastError());
+  SEM_TRY(launch_gsmap_masked(st, C.mesh->map, C.mesh->layout(), C.op->desc.bc_bits, C.b, C.b));
+  SEM_TRY(vcycle(ctx, mg, l + 1));
+  const long n = (long)L.mesh->nl;
+  hipLaunchKernelGGL(k_prolong, dim3(nblk(n)), dim3(256), 0, st, C.x, L.J, L.e, L.x, L.xf, n);
+  SEM_HIP(hipGetLastError());
+  SEM_TRY(apply(ctx, L.op, L.e, L.w));
+  hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
+  SEM_HIP(hipGetLastError());
+  return cheby(ctx, mg, L, L.b, 0, mg->prm.degree, false);
+}
+
+// lmax = safety x Rayleigh quotient after npow power steps of D^-1 A
+int estimate_lmax(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& L) {
+  hipStream_t st = ctx->stream;
+  const sem_mesh* m = L.mesh;
+  const long n = (long)m->nl;
+  const Layout g = m->layout();
+  double* v = L.d;
+  double* w = L.w;
+  hipLaunchKernelGGL(k_hash_start, dim3(nblk(n)), dim3(256), 0, st, m->l2g, v, m->L, m->NC,
+                     L.op->desc.bc_bits, n);
+  SEM_HIP(hipGetLastError());
+  for (int it = 0; it < mg->prm.npow; ++it) {
+    SEM_TRY(apply(ctx, L.op, v, w));
+    hipLaunchKernelGGL(k_mul2, dim3(nblk(n)), dim3(256), 0, st, L.dinv, w, L.e, n);
+    SEM_TRY(launch_norm_inf(st, n, L.e, mg->part, kParts, mg->part + kParts));
+    hipLaunchKernelGGL(k_div_scalar, dim3(nblk(n)), dim3(256), 0, st, L.e, mg->part + kParts, v,
+                       n);
+    SEM_HIP(hipGetLastError());
+  }
+  double num = 0.0, den = 0.0;
+  SEM_TRY(apply(ctx, L.op, v, w));
+  SEM_TRY(wdot_host(ctx, mg, g, v, w, &num));
+  hipLaunchKernelGGL(k_times_diag, dim3(nblk(n)), dim3(256), 0, st, v, L.dinv, L.e, n);
+  SEM_TRY(wdot_host(ctx, mg, g, v, L.e, &den));
+  if (!(den > 0.0) || !(num > 0.0) || !std::isfinite(num / den)) {
+    set_error("sem_mg_create: eigenvalue estimate failed on a level with p = %d, %d x %d "
+              "elements (Rayleigh quotient %g / %g)", m->nr, m->Ex, m->Ey, num, den);
+    return SEM_ERR_BREAKDOWN;
+  }
+  L.lmax = mg->prm.safety * num / den;
+  return SEM_OK;
+}
+
+int level_buffers(sem_mg::Level& L, bool own_xb) {
+  const long n = (long)L.mesh->nl;
+  SEM_TRY(dalloc(&L.dinv, n));
+  SEM_TRY(dalloc(&L.r, n));
+  SEM_TRY(dalloc(&L.d, n));
+  SEM_TRY(dalloc(&L.w, n));
+  SEM_TRY(dalloc(&L.e, n));
+  if (own_xb) {
+    SEM_TRY(dalloc(&L.x, n));
+    SEM_TRY(dalloc(&L.b, n));
+  }
+  return SEM_OK;
+}
+
+int build(sem_ctx* ctx, sem_mg* mg) {
+  hipStream_t st = ctx->stream;
+  const sem_operator* op0 = mg->op;
+  {
+    sem_mg::Level L0;
+    L0.mesh = const_cast<sem_mesh*>(op0->mesh);
+    L0.op = op0;
+    mg->lv.push_back(L0);
+  }
+  while ((int)mg->lv.size() < mg->prm.max_levels) {
+    sem_mg::Level& F = mg->lv.back();
+    const sem_mesh* mf = F.mesh;
+    const int N = mf->nr - 1;
+    int pc, h;
+    if (N > 1) {
+      pc = N / 2 + 1;
+      h = 0;
+    } else if (mf->Ex % 2 == 0 && mf->Ey % 2 == 0 && mf->Ex >= 4 && mf->Ey >= 4) {
+      pc = 2;
+      h = 1;
+    } else {
+      break;
+    }
+    const int pf = mf->nr;
+    std::vector<double> zc(pc), wc(pc);
+    gll(pc, zc.data(), wc.data());
+    sem_mesh* mc = nullptr;
+    SEM_TRY(sem_mesh_create(ctx, pc, pc, mf->Ex >> h, mf->Ey >> h, mf->per_x, mf->per_y,
+                            SEM_DEFORM_NONE, 0.0, &mc));
+    sem_mg::Level C;
+    C.mesh = mc;
+    C.own_mesh = true;
+    mg->lv.push_back(C);
+    sem_mg::Level& Fl = mg->lv[mg->lv.size() - 2];
+    Fl.h = h;
+    Xfer xf{};
+    xf.pf = pf;
+    xf.pc = pc;
+    xf.hx = h;
+    xf.hy = h;
+    xf.Lf = mf->L;
+    xf.NCf = mf->NC;
+    xf.Lc = mc->L;
+    xf.NCc = mc->NC;
+    xf.per_x = mf->per_x;
+    xf.per_y = mf->per_y;
+    Fl.xf = xf;
+    // transfer rows: fine GLL nodes of each sub-element in the coarse
+    // reference coordinate, for x then y (square elements: identical)
+    const std::vector<double>& zf = mf->hzr;
+    std::vector<double> Jall;
+    for (int dir = 0; dir < 2; ++dir)
+      for (int s = 0; s < (h ? 2 : 1); ++s) {
+        std::vector<double> xo(pf);
+        for (int i = 0; i < pf; ++i) xo[i] = h ? (zf[i] + 2.0 * s - 1.0) / 2.0 : zf[i];
+        std::vector<double> J = interp_rows(pf, xo.data(), pc, zc.data());
+        Jall.insert(Jall.end(), J.begin(), J.end());
+      }
+    SEM_TRY(dalloc(&Fl.J, (long)Jall.size()));
+    SEM_HIP(hipMemcpy(Fl.J, Jall.data(), sizeof(double) * Jall.size(), hipMemcpyHostToDevice));
+    // coarse coordinates: the fine geometry at the coarse nodes
+    int subbits = 0;
+    std::vector<double> xo(pc);
+    for (int a = 0; a < pc; ++a) {
+      const int sub = (h && zc[a] > 0.0) ? 1 : 0;
+      subbits |= sub << a;
+      xo[a] = h ? 2.0 * zc[a] + 1.0 - 2.0 * sub : zc[a];
+    }
+    std::vector<double> Jc = interp_rows(pc, xo.data(), pf, zf.data());
+    double* dJc = nullptr;
+    SEM_TRY(dalloc(&dJc, (long)Jc.size()));
+    SEM_HIP(hipMemcpy(dJc, Jc.data(), sizeof(double) * Jc.size(), hipMemcpyHostToDevice));
+    const long nc = (long)mc->nl;
+    hipLaunchKernelGGL(k_coarse_coords, dim3(nblk(nc)), dim3(256), 0, st, mf->f[SEM_X], dJc,
+                       subbits, mc->f[SEM_X], xf, nc);
+    hipLaunchKernelGGL(k_coarse_coords, dim3(nblk(nc)), dim3(256), 0, st, mf->f[SEM_Y], dJc,
+                       subbits, mc->f[SEM_Y], xf, nc);
+    const hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(dJc);
+    SEM_HIP(e);
+    SEM_TRY(sem_mesh_set_coords(ctx, mc, mc->f[SEM_X], mc->f[SEM_Y]));
+  }
+  // operators, diagonals, buffers, eigenvalue estimates
+  for (size_t l = 0; l < mg->lv.size(); ++l) {
+    sem_mg::Level& L = mg->lv[l];
+    if (l > 0) {
+      sem_op_desc d{};
+      d.nu = op0->desc.nu;
+      d.k = op0->desc.k;
+      d.bc_bits = op0->desc.bc_bits;
+      d.assemble = 1;
+      sem_operator* oc = nullptr;
+      SEM_TRY(sem_operator_create(ctx, L.mesh, &d, SEM_F64, &oc));
+      L.op = oc;
+      L.own_op = true;
+    }
+    SEM_TRY(level_buffers(L, l > 0));
+    SEM_TRY(sem_operator_jacobi(ctx, L.op, L.dinv));
+    SEM_TRY(estimate_lmax(ctx, mg, L));   // uses the level's d, w, e only
+  }
+  const long n0 = (long)op0->mesh->nl;
+  SEM_TRY(dalloc(&mg->pr, n0));
+  SEM_TRY(dalloc(&mg->pp, n0));
+  SEM_TRY(dalloc(&mg->pw, n0));
+  SEM_TRY(dalloc(&mg->pz, n0));
+  SEM_HIP(hipStreamSynchronize(st));
+  return SEM_OK;
+}
+
+int check_op(const sem_operator* op) {
+  SEM_ARG(op->precision == SEM_F64, "multigrid: the operator must be SEM_F64");
+  SEM_ARG(op->desc.assemble, "multigrid: the operator must assemble (desc.assemble = 1)");
+  if (op->desc.nu_f || op->desc.k_f || op->desc.G11 || op->desc.G12 || op->desc.G22 ||
+      op->desc.B) {
+    set_error("multigrid: field nu/k and factor overrides are not supported (scalar nu, k only)");
+    return SEM_ERR_UNSUPPORTED;
+  }
+  return SEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem_mg_default_params(sem_mg_params* p) {
+  SEM_ARG(p, "NULL argument");
+  p->degree = 3;
+  p->ratio = 20.0;
+  p->degree_coarse = 8;
+  p->npow = 20;
+  p->safety = 1.1;
+  p->max_levels = 32;
+  return SEM_OK;
+}
+
+int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* params,
+                  sem_mg** out) {
+  SEM_ARG(ctx && op && out, "NULL argument");
+  SEM_TRY(check_op(op));
+  sem_mg_params p;
+  sem_mg_default_params(&p);
+  if (params) p = *params;
+  SEM_ARG(p.degree >= 1 && p.degree_coarse >= 1 && p.ratio > 1.0 && p.npow >= 1 &&
+              p.safety >= 1.0 && p.max_levels >= 1,
+          "sem_mg_create: bad parameters");
+  sem_mg* mg = new (std::nothrow) sem_mg();
+  SEM_ARG(mg, "out of host memory");
+  mg->prm = p;
+  mg->device = ctx->device;
+  mg->op = op;
+  if (dalloc(&mg->part, kParts + 1) != SEM_OK) {
+    mg_free(mg);
+    return SEM_ERR_HIP;
+  }
+  const int s = build(ctx, mg);
+  if (s != SEM_OK) {
+    mg_free(mg);
+    return s;
+  }
+  *out = mg;
+  return SEM_OK;
+}
+
+int sem_mg_destroy(sem_mg* mg) {
+  mg_free(mg);
+  return SEM_OK;
+}
+
+int sem_mg_num_levels(const sem_mg* mg, int* nlevels) {
+  SEM_ARG(mg && nlevels, "NULL argument");
+  *nlevels = (int)mg->lv.size();
+  return SEM_OK;
+}
+
+int sem_mg_level_info(const sem_mg* mg, int level, int* p, int* Ex, int* Ey, double* lmax) {
+  SEM_ARG(mg, "NULL argument");
+  SEM_ARG(level >= 0 && level < (int)mg->lv.size(), "sem_mg_level_info: no level %d", level);
+  const sem_mg::Level& L = mg->lv[level];
+  if (p) *p = L.mesh->nr;
+  if (Ex) *Ex = L.mesh->Ex;
+  if (Ey) *Ey = L.mesh->Ey;
+  if (lmax) *lmax = L.lmax;
+  return SEM_OK;
+}
+
+int sem_mg_set_lmax(sem_mg* mg, int level, double lmax) {
+  SEM_ARG(mg, "NULL argument");
+  SEM_ARG(level >= 0 && level < (int)mg->lv.size(), "sem_mg_set_lmax: no level %d", level);
+  SEM_ARG(lmax > 0.0 && std::isfinite(lmax), "sem_mg_set_lmax: lmax must be positive");
+  mg->lv[level].lmax = lmax;
+  return SEM_OK;
+}
+
+int sem_mg_vcycle(sem_ctx* ctx, sem_mg* mg, const double* r, double* z) {
+  SEM_ARG(ctx && mg && r && z, "NULL argument");
+  SEM_ARG(r != z, "sem_mg_vcycle: r and z must not alias");
+  sem_mg::Level& L0 = mg->lv[0];
+  L0.b = const_cast<double*>(r);
+  L0.x = z;
+  const int s = vcycle(ctx, mg, 0);
+  L0.b = nullptr;
+  L0.x = nullptr;
+  return s;
+}
+
+int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b, double* x,
+               double tol, int maxiter, sem_pcg_info* info) {
+  SEM_ARG(ctx && op && mg && b && x, "NULL argument");
+  SEM_ARG(op == mg->op, "sem_pcg_mg: mg was built for another operator");
+  SEM_ARG(b != x, "sem_pcg_mg: b and x must not alias");
+  SEM_ARG(maxiter >= 0 && tol >= 0.0, "sem_pcg_mg: bad limits");
+  hipStream_t st = ctx->stream;
+  const sem_mesh* m = op->mesh;
+  const Layout g = m->layout();
+  const long n = (long)m->nl;
+  SEM_HIP(hipEventRecord(ctx->ev0, st));
+  SEM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  SEM_HIP(hipMemcpyAsync(mg->pr, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  double res = 0.0, t = 0.0, t_prev = 1.0;
+  SEM_TRY(norm_host(ctx, mg, n, mg->pr, &res));
+  int k = 0;
+  bool conv = true;
+  while (res > tol) {
+    if (k == maxiter) {
+      conv = false;
+      break;
+    }
+    ++k;
+    SEM_TRY(sem_mg_vcycle(ctx, mg, mg->pr, mg->pz));
+    SEM_TRY(wdot_host(ctx, mg, g, mg->pr, mg->pz, &t));
+    if (k == 1) {
+      SEM_HIP(hipMemcpyAsync(mg->pp, mg->pz, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    } else {
+      SEM_TRY(launch_axpby(st, n, 1.0, mg->pz, t / t_prev, mg->pp, mg->pp));
+    }
+    SEM_TRY(apply(ctx, op, mg->pp, mg->pw));
+    double pw = 0.0;
+    SEM_TRY(wdot_host(ctx, mg, g, mg->pp, mg->pw, &pw));
+    const double alpha = t / pw;
+    hipLaunchKernelGGL(k_xr, dim3(nblk(n)), dim3(256), 0, st, x, mg->pr, mg->pp, mg->pw, alpha, n);
+    SEM_HIP(hipGetLastError());
+    SEM_TRY(norm_host(ctx, mg, n, mg->pr, &res));
+    t_prev = t;
+    if (!std::isfinite(res)) break;
+  }
+  SEM_HIP(hipEventRecord(ctx->ev1, st));
+  SEM_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  SEM_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (info) {
+    info->iters = k;
+    info->converged = conv && res <= tol;
+    info->res = res;
+    info->elapsed_ms = ms;
+    info->outer = 0;
+    info->inner_total = k;
+  }
+  if (!std::isfinite(res)) {
+    set_error("sem_pcg_mg: residual is not finite after %d iterations", k);
+    return SEM_ERR_NONFINITE;
+  }
+  return (conv && res <= tol) ? SEM_OK : SEM_WARN_MAXITER;
+}
+
+}  // extern "C"

@@ -123,10 +123,13 @@ __global__ void k_mul(const double* __restrict__ a, const double* __restrict__ u
 }
 
 // Map-based QQ': one thread per global node, copies summed pairwise in map
-// order ((v0+v1)+(v2+v3)), the sum written back to every copy.
+// order ((v0+v1)+(v2+v3)), the sum written back to every copy. u and y may be
+// the same array (every copy of a node is read before any is written, and
+// nodes are disjoint). bc >= 0 also applies the Dirichlet mask of the
+// structured layout (L rows, NC columns), a property of the global node.
 template <typename T>
 __global__ void k_gsmap(const int32_t* __restrict__ perm, const int32_t* __restrict__ seg,
-                        const T* __restrict__ u, T* __restrict__ y, long ng) {
+                        const T* u, T* y, long ng, int bc, int L, int NC) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   const int s0 = seg[g], s1 = seg[g + 1];
@@ -149,6 +152,10 @@ __global__ void k_gsmap(const int32_t* __restrict__ perm, const int32_t* __restr
       first = false;
     }
     if (k < s1) s = first ? u[perm[k]] : s + u[perm[k]];
+  }
+  if (bc >= 0) {
+    const int l = perm[s0];
+    s = s * (mask_on(l % L, l / L, L, NC, bc) ? T(1) : T(0));
   }
   for (int k = s0; k < s1; ++k) y[perm[k]] = s;
 }
@@ -622,14 +629,22 @@ int launch_mul(hipStream_t st, int64_t n, const double* a, const double* u, doub
 
 int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* y) {
   hipLaunchKernelGGL((k_gsmap<double>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
-                     map->seg, u, y, (long)map->n_global);
+                     map->seg, u, y, (long)map->n_global, -1, 1, 1);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_gsmap_masked(hipStream_t st, const sem_gsmap* map, const Layout& g, int bits,
+                        const double* u, double* y) {
+  hipLaunchKernelGGL((k_gsmap<double>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
+                     map->seg, u, y, (long)map->n_global, bits, g.L, g.NC);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }
 
 int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y) {
   hipLaunchKernelGGL((k_gsmap<float>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
-                     map->seg, u, y, (long)map->n_global);
+                     map->seg, u, y, (long)map->n_global, -1, 1, 1);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

@@ -38,6 +38,11 @@ class sem_op_desc(C.Structure):
                 ("bc_bits", C.c_int), ("assemble", C.c_int)]
 
 
+class sem_mg_params(C.Structure):
+    _fields_ = [("degree", C.c_int), ("ratio", C.c_double), ("degree_coarse", C.c_int),
+                ("npow", C.c_int), ("safety", C.c_double), ("max_levels", C.c_int)]
+
+
 class sem_pcg_info(C.Structure):
     _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("res", C.c_double),
                 ("elapsed_ms", C.c_double), ("outer", C.c_int), ("inner_total", C.c_int)]
@@ -87,6 +92,14 @@ SIGNATURES = {
     "sem_operator_jacobi": [_vp, _vp, _vp],
     "sem_pcg": [_vp, _vp, _i, _vp, _vp, _vp, _d, _i, C.POINTER(sem_pcg_info)],
     "sem_pcg_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _i, C.POINTER(sem_pcg_info)],
+    "sem_mg_default_params": [C.POINTER(sem_mg_params)],
+    "sem_mg_create": [_vp, _vp, C.POINTER(sem_mg_params), _pvp],
+    "sem_mg_destroy": [_vp],
+    "sem_mg_num_levels": [_vp, C.POINTER(_i)],
+    "sem_mg_level_info": [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _dp],
+    "sem_mg_set_lmax": [_vp, _i, _d],
+    "sem_mg_vcycle": [_vp, _vp, _vp, _vp],
+    "sem_pcg_mg": [_vp, _vp, _vp, _vp, _vp, _d, _i, C.POINTER(sem_pcg_info)],
     "sem_wdot": [_vp, _vp, _vp, _vp, _dp],
     "sem_norm_inf": [_vp, _i64, _vp, _dp],
     "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],

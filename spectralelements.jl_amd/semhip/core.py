@@ -713,6 +713,50 @@ class Diagonal:
     __mul__ = __call__
 
 
+class MG:
+    """Multigrid V-cycle preconditioner for an SEMOperator (a stronger opM
+    than the identity / Jacobi of diffusion.jl:47-49): p-multigrid on the
+    element grid, then h-multigrid on bilinear elements, Chebyshev-Jacobi
+    smoothing. ``mg(r)`` applies one V-cycle to an assembled residual;
+    ``pcg(b, op, opM=mg)`` runs PCG with it on the GPU (sem_pcg_mg). The
+    operator must be FP64, assembled, with scalar nu and k."""
+
+    def __init__(self, op: SEMOperator, degree: int = 3, ratio: float = 20.0,
+                 degree_coarse: int = 8, npow: int = 20, safety: float = 1.1,
+                 max_levels: int = 32):
+        self.op = op
+        prm = L.sem_mg_params(int(degree), float(ratio), int(degree_coarse), int(npow),
+                              float(safety), int(max_levels))
+        h = C.c_void_p()
+        L.call("sem_mg_create", ctx(), op.h, C.byref(prm), C.byref(h))
+        self.h = h
+        self._fin = weakref.finalize(self, L.load().sem_mg_destroy, h)
+
+    def levels(self):
+        """[(p, Ex, Ey, lmax)] from the finest level to the coarsest."""
+        n = C.c_int()
+        L.call("sem_mg_num_levels", self.h, C.byref(n))
+        out = []
+        for lv in range(n.value):
+            p, ex, ey, lm = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+            L.call("sem_mg_level_info", self.h, lv, C.byref(p), C.byref(ex), C.byref(ey),
+                   C.byref(lm))
+            out.append((p.value, ex.value, ey.value, lm.value))
+        return out
+
+    def set_lmax(self, level: int, lmax: float):
+        L.call("sem_mg_set_lmax", self.h, int(level), float(lmax))
+
+    def __call__(self, r, out=None):
+        was_np = isinstance(r, np.ndarray)
+        r = to_device(r)
+        z = out if out is not None else _like(r)
+        L.call("sem_mg_vcycle", ctx(), self.h, _p(r), _p(z))
+        return _wrap_out(z, was_np)
+
+    __mul__ = __call__
+
+
 class PCGInfo:
     def __init__(self):
         self.iters = 0
@@ -736,12 +780,30 @@ identity._sem_identity = True
 def pcg(b, opA, opM=None, mult=None, ifv: bool = False, tol: float = 1e-8,
         maxiter: Optional[int] = None, info: Optional[PCGInfo] = None, x=None):
     """Left-preconditioned CG from x0 = 0 (pcg.jl:16-60): stops when
-    norm(r, Inf) <= tol, inner products weighted by mult, warns and returns
-    at k == maxiter. An SEMOperator with an identity or Diagonal opM (and the
-    operator mesh's mult) runs fully fused on the GPU."""
+    norm(r, Inf) <= tol, inner products weighted by mult (None: unweighted,
+    the reference's mult = ones), warns and returns at k == maxiter. An
+    SEMOperator with the operator mesh's mult and an identity, Diagonal or MG
+    opM runs fused on the GPU; anything else runs the same recursion with
+    device vector algebra."""
     was_np = isinstance(b, np.ndarray)
     info = info if info is not None else PCGInfo()
-    if isinstance(opA, SEMOperator) and (mult is None or _same_mult(mult, opA.msh)) and (
+    if isinstance(opA, SEMOperator) and isinstance(opM, MG) and opM.op is opA and (
+            _same_mult(mult, opA.msh)):
+        bd = to_device(b)
+        if maxiter is None:
+            maxiter = bd.numel()
+        xd = x if x is not None else _like(bd)
+        pi = L.sem_pcg_info()
+        st = L.call("sem_pcg_mg", ctx(), opA.h, opM.h, _p(bd), _p(xd), float(tol),
+                    int(min(maxiter, 2**31 - 1)), C.byref(pi))
+        info.iters, info.res, info.converged = pi.iters, pi.res, bool(pi.converged)
+        info.elapsed_ms = pi.elapsed_ms
+        if st == L.SEM_WARN_MAXITER:
+            L.maxiter_warning(pi.res)
+        if ifv:
+            print(f"PCG iter: {pi.iters}, res: {pi.res}")
+        return _wrap_out(xd, was_np)
+    if isinstance(opA, SEMOperator) and _same_mult(mult, opA.msh) and (
             _is_identity(opM) or isinstance(opM, Diagonal)):
         bd = to_device(b, dtype=opA.dtype)
         if maxiter is None:
@@ -766,7 +828,17 @@ def pcg(b, opA, opM=None, mult=None, ifv: bool = False, tol: float = 1e-8,
 
 
 def _same_mult(mult, msh) -> bool:
-    return mult is msh._cache.get("mult") or mult is getattr(msh, "_mult_marker", None)
+    """True when mult holds the mesh's own 1/multiplicity (the fused solvers
+    weight their inner products with it structurally). None is the
+    reference's default mult = ones (pcg.jl:18), which is not."""
+    if mult is None:
+        return False
+    if mult is msh._cache.get("mult"):
+        return True
+    if tuple(getattr(mult, "shape", ())) != tuple(msh.shape):
+        return False
+    m = mult if isinstance(mult, torch.Tensor) else torch.as_tensor(np.asarray(mult))
+    return bool(torch.equal(m.to(device=msh.mult.device, dtype=_F64), msh.mult))
 
 
 def _pcg_generic(b, opA, opM, mult, ifv, tol, maxiter, info, x, was_np):

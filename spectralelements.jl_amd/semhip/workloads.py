@@ -60,3 +60,11 @@ def gaussian_blob(m: Mesh, x0: float = 0.5, y0: float = 0.0, sigma: float = 0.1)
     out = empty_field(*m.shape)
     out.copy_(u)
     return out
+
+
+def helmholtz_rhs(se, m: Mesh, M, seed: int) -> torch.Tensor:
+    """b = QQ'(M .* (B .* f)) for f ~ N(0, 1) per global node: the assembled
+    right-hand side of the benchmark solves (makeRHS!, diffusion.jl:51-65,
+    with a random forcing)."""
+    f = global_normal_field(m, seed)
+    return se.gatherScatter(se.mask(se.mass(f, m), M), m)

@@ -300,7 +300,8 @@ def test_f32_operator_and_mixed_solve(se):
                                 info=info))
     assert info.converged
     info64 = se.PCGInfo()
-    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), tol=tol, info=info64))
+    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), mult=m.mult, tol=tol,
+                            info=info64))
     r = b - o.opLHS(x, 1.0, 1.0, mo, Mo)
     assert np.abs(r).max() <= 1.01 * tol
     assert _rel(x, x64) < 1e-8
@@ -436,7 +437,7 @@ def test_mixed_solve_reaches_fp64_tolerance(se, prec):
     assert info.converged, (prec, info.outer, info.res)
     r = b - o.opLHS(x, 1.0, 1.0, mo, Mo)
     assert np.abs(r).max() <= 1.01 * tol
-    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), tol=tol))
+    x64 = se.to_host(se.pcg(se.to_device(b), op64, opM=se.Diagonal(d), mult=m.mult, tol=tol))
     assert _rel(x, x64) < 1e-8
 
 
