@@ -221,6 +221,20 @@ typedef struct sem_pcg_info {
 int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
             const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
 
+/* sem_pcg without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device or pinned host memory during a solve
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels, results and status codes as sem_pcg;
+ * its workspace (r, p, w, reduction partials, device scalars) is a fixed
+ * device array of 24 MiB + 128 KiB per GPU, held until the process ends, and
+ * the device state is read back by plain copies. It fits n_local up to about
+ * 1,048,000 FP64 (2,096,000 FP32) local nodes; larger meshes return
+ * SEM_ERR_UNSUPPORTED before any launch. Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (several streams or threads) race;
+ * each call clears what must start at zero. */
+int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
+                    const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
+
 /* Mixed precision: FP64 defect correction by residual replacement ("reliable
  * updates"). One fused PCG runs with the low-precision operator op_lo on FP32
  * vectors; its iterate is accumulated into x in FP64 and its recursive

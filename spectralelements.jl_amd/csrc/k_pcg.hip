@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "sem_device.h"
 
@@ -394,7 +395,8 @@ struct PcgRun {
   PcgState* state = nullptr;
   PcgState* hs = nullptr;
 
-  int setup(sem_ctx* c, const sem_operator* o, const T* dd) {
+  // launch geometry; returns the workspace bytes the solve needs
+  size_t plan(sem_ctx* c, const sem_operator* o, const T* dd) {
     ctx = c;
     op = o;
     d = dd;
@@ -406,23 +408,36 @@ struct PcgRun {
     nQ = pl.nk1;
     sg = SeamGeom{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x,
                   g.per_y, op->desc.bc_bits};
-    // workspace: r, p, w | part[2][2*n3] | Q[nQ] | state
-    const size_t vb = ((sizeof(T) * n + 255) / 256) * 256;
-    const size_t pb = sizeof(double) * 4 * n3;
-    const size_t qb = ((sizeof(double) * nQ + 255) / 256) * 256;
-    SEM_TRY(ensure_ws(ctx, 3 * vb + pb + qb + 256));
-    char* w0 = static_cast<char*>(ctx->ws);
+    return 3 * vec_bytes() + part_bytes() + q_bytes() + 256;
+  }
+  size_t vec_bytes() const { return ((sizeof(T) * n + 255) / 256) * 256; }
+  size_t part_bytes() const { return sizeof(double) * 4 * n3; }
+  size_t q_bytes() const { return ((sizeof(double) * nQ + 255) / 256) * 256; }
+
+  // binds the workspace (r, p, w | part[2][2*n3] | Q[nQ] | state) and the
+  // host copy of the state
+  void bind(void* ws, PcgState* host_state) {
+    const size_t vb = vec_bytes(), pb = part_bytes(), qb = q_bytes();
+    char* w0 = static_cast<char*>(ws);
     r = reinterpret_cast<T*>(w0);
     p = reinterpret_cast<T*>(w0 + vb);
     w = reinterpret_cast<T*>(w0 + 2 * vb);
     part = reinterpret_cast<double*>(w0 + 3 * vb);
     Q = reinterpret_cast<double*>(w0 + 3 * vb + pb);
     state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb);
-    SEM_TRY(ensure_pinned(ctx, 128));  // PcgState, then a scalar slot at byte 64
-    hs = static_cast<PcgState*>(ctx->pinned);
+    hs = host_state;
     aa = args_of(op, nullptr, w);   // factors, precision, recomputed geometry
     aa.bc = op->desc.bc_bits;
     aa.assemble = 1;
+  }
+
+  // workspace from the context (grown on demand), state read back through
+  // the context's pinned buffer
+  int setup(sem_ctx* c, const sem_operator* o, const T* dd) {
+    const size_t bytes = plan(c, o, dd);
+    SEM_TRY(ensure_ws(ctx, bytes));
+    SEM_TRY(ensure_pinned(ctx, 128));  // PcgState, then a scalar slot at byte 64
+    bind(ctx->ws, static_cast<PcgState*>(ctx->pinned));
     return SEM_OK;
   }
 
@@ -520,10 +535,20 @@ struct PcgRun {
 };
 
 template <typename T>
+int pcg_solve(PcgRun<T>& R, sem_ctx* ctx, const sem_operator* op, const T* b, T* x, double tol,
+              int maxiter, sem_pcg_info* info);
+
+template <typename T>
 int pcg_run_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x, double tol,
               int maxiter, sem_pcg_info* info) {
   PcgRun<T> R;
   SEM_TRY(R.setup(ctx, op, d));
+  return pcg_solve(R, ctx, op, b, x, tol, maxiter, info);
+}
+
+template <typename T>
+int pcg_solve(PcgRun<T>& R, sem_ctx* ctx, const sem_operator* op, const T* b, T* x, double tol,
+              int maxiter, sem_pcg_info* info) {
   hipStream_t st = ctx->stream;
   SEM_HIP(hipEventRecord(ctx->ev0, st));
   SEM_TRY(R.init(b, x));
@@ -765,6 +790,67 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
   return status;
 }
 
+// ---- allocation-free PCG: the workspace of sem_pcg (r, p, w, the block
+// partials, the <p, Ap> partials and the device state) in a fixed device
+// array instead of the context's grown-on-demand buffer. One array per GPU,
+// shared by every call on that GPU.
+constexpr size_t kScratchBytes = (size_t(24) << 20) + (size_t(128) << 10);
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+// Device address of this GPU's scratch array, looked up once per device with
+// that device current (sem_ctx_create calls this after hipSetDevice).
+int init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d outside the %d devices the scratch table holds", device, kMaxDevices);
+    return SEM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return SEM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+              hipGetErrorString(e));
+    return SEM_ERR_HIP;
+  }
+  g_scratch_ptr[device] = p;
+  return SEM_OK;
+}
+
+namespace {
+
+template <typename T>
+int pcg_noalloc_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, T* x, double tol,
+                  int maxiter, sem_pcg_info* info) {
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ws = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  }
+  if (!ws) {
+    set_error("sem_pcg_noalloc: the scratch array of device %d is not set up", ctx->device);
+    return SEM_ERR_HIP;
+  }
+  PcgRun<T> R;
+  const size_t need = R.plan(ctx, op, d);
+  if (need > kScratchBytes) {
+    set_error("sem_pcg_noalloc: %lld local nodes need %zu bytes of scratch, more than its %zu "
+              "(at most about %zu local nodes in %s)",
+              (long long)op->mesh->nl, need, kScratchBytes,
+              (kScratchBytes - (size_t(128) << 10)) / (3 * sizeof(T)),
+              sizeof(T) == 8 ? "FP64" : "FP32");
+    return SEM_ERR_UNSUPPORTED;
+  }
+  PcgState hstate{};   // read back by a plain copy: no pinned buffer
+  R.bind(ws, &hstate);
+  return pcg_solve(R, ctx, op, b, x, tol, maxiter, info);
+}
+
 }  // namespace
 
 int launch_jacobi_local(hipStream_t st, const sem_operator* op, double* dl) {
@@ -802,6 +888,24 @@ int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, co
 }  // namespace sem
 
 using namespace sem;
+
+extern "C" int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
+                               const void* b, void* x, double tol, int maxiter,
+                               sem_pcg_info* info) {
+  SEM_ARG(ctx && op && b && x, "NULL argument");
+  SEM_ARG(op->desc.assemble, "sem_pcg_noalloc needs an assembled operator (desc.assemble = 1)");
+  SEM_ARG(precond == SEM_PRECOND_IDENTITY || (precond == SEM_PRECOND_DIAG && d),
+          "sem_pcg_noalloc: SEM_PRECOND_DIAG needs the diagonal d");
+  SEM_ARG(maxiter >= 0, "sem_pcg_noalloc: maxiter < 0");
+  SEM_ARG(tol >= 0.0, "sem_pcg_noalloc: tol < 0");
+  if (precond == SEM_PRECOND_IDENTITY) d = nullptr;
+  if (op->precision == SEM_F64)
+    return pcg_noalloc_t<double>(ctx, op, static_cast<const double*>(d),
+                                 static_cast<const double*>(b), static_cast<double*>(x), tol,
+                                 maxiter, info);
+  return pcg_noalloc_t<float>(ctx, op, static_cast<const float*>(d), static_cast<const float*>(b),
+                              static_cast<float*>(x), tol, maxiter, info);
+}
 
 extern "C" int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
                              const double* d64, const double* b, double* x, double tol,

@@ -235,6 +235,9 @@ int launch_adjoint_dnu(hipStream_t st, const sem_mesh* m, const double* u, const
 int launch_advect(hipStream_t st, const sem_mesh* mV, const sem_mesh* mD, const double* J,
                   const double* T, const double* vx, const double* vy, double* y, int transpose);
 
+// Looks up this GPU's scratch array of the allocation-free PCG (device current)
+int init_scratch(int device);
+
 // PCG driver (k_pcg.hip)
 int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,
             void* x, double tol, int maxiter, sem_pcg_info* info);

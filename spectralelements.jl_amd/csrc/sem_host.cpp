@@ -209,6 +209,7 @@ int sem_ctx_create(int device, void* stream, sem_ctx** out) {
   SEM_HIP(hipGetDeviceCount(&ndev));
   SEM_ARG(device >= 0 && device < ndev, "sem_ctx_create: device %d not in [0, %d)", device, ndev);
   SEM_HIP(hipSetDevice(device));
+  SEM_TRY(init_scratch(device));
   sem_ctx* c = new (std::nothrow) sem_ctx();
   SEM_ARG(c, "out of host memory");
   c->device = device;

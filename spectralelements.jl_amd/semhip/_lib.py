@@ -91,6 +91,7 @@ SIGNATURES = {
     "sem_operator_apply": [_vp, _vp, _vp, _vp],
     "sem_operator_jacobi": [_vp, _vp, _vp],
     "sem_pcg": [_vp, _vp, _i, _vp, _vp, _vp, _d, _i, C.POINTER(sem_pcg_info)],
+    "sem_pcg_noalloc": [_vp, _vp, _i, _vp, _vp, _vp, _d, _i, C.POINTER(sem_pcg_info)],
     "sem_pcg_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _i, C.POINTER(sem_pcg_info)],
     "sem_mg_default_params": [C.POINTER(sem_mg_params)],
     "sem_mg_create": [_vp, _vp, C.POINTER(sem_mg_params), _pvp],
