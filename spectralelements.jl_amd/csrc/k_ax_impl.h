@@ -291,6 +291,19 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
     const int base = i + L * jb;
     const int bc_ = ic + L * jb;
     T uv[P], us[P];
+    // the row's geometric factors, issued right after its other loads so
+    // that all of them are in flight together (their use is two phases later)
+    T g11[kEarlyG ? P : 1], g12[kEarlyG ? P : 1], g22[kEarlyG ? P : 1];
+    auto load_g = [&]() {
+      if (kEarlyG) {
+#pragma unroll
+        for (int b = 0; b < (kEarlyG ? P : 1); ++b) {
+          g11[b] = prm.G11[bc_ + L * b];
+          g12[b] = prm.G12[bc_ + L * b];
+          g22[b] = prm.G22[bc_ + L * b];
+        }
+      }
+    };
     // R1: load row a of element e, s-contraction us(a,b) = sum_k u(a,k) Ds_{b,k}
     if (MODE == 2) {
       T rv[P], zv[P];
@@ -329,6 +342,7 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
     } else {
 #pragma unroll
       for (int b = 0; b < P; ++b) uv[b] = prm.u[bc_ + L * b];
+      load_g();
     }
     if constexpr (MF == 0) {
 #pragma unroll
@@ -341,15 +355,9 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
     }
 #pragma unroll
     for (int b = 0; b < P; ++b) s0[b * RS + tid] = uv[b];
-    T g11[kEarlyG ? P : 1], g12[kEarlyG ? P : 1], g22[kEarlyG ? P : 1];
-    if (kEarlyG) {
-#pragma unroll
-      for (int b = 0; b < (kEarlyG ? P : 1); ++b) {
-        g11[b] = prm.G11[bc_ + L * b];
-        g12[b] = prm.G12[bc_ + L * b];
-        g22[b] = prm.G22[bc_ + L * b];
-      }
-    }
+    // PCG mode: after the branches of R1 (a load issued before a branch
+    // would be waited for where its paths join)
+    if (MODE == 2) load_g();
     __syncthreads();
     if constexpr (MF != 0) {
       // M1: per element (wave wv takes elements wv, wv+4, ...):
@@ -437,10 +445,15 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
         }
       }
     }
+    // B is loaded unconditionally (G22 stands in without a mass term: those
+    // lines were just read, so they come from cache): a load under a runtime
+    // branch would make the compiler wait for every outstanding load at the
+    // join, before the barrier, instead of letting B's latency hide under C2
     T bv[P];
-    if (prm.has_mass && plain) {
+    {
+      const T* bsrc = (prm.has_mass && plain) ? prm.B : prm.G22;
 #pragma unroll
-      for (int b = 0; b < P; ++b) bv[b] = prm.B[bc_ + L * b];
+      for (int b = 0; b < P; ++b) bv[b] = bsrc[bc_ + L * b];
     }
     __syncthreads();
     if constexpr (MF != 0) {

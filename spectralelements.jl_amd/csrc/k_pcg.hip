@@ -30,7 +30,7 @@ namespace {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
+k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
            const T* __restrict__ d, int P, int L, int NC, int per_x, int per_y, long n,
            double* __restrict__ part, int n3) {
   __shared__ double red[256];
@@ -42,6 +42,7 @@ k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
     const T bv = b[t];
     x[t] = T(0);
     r[t] = bv;
+    p[t] = T(0);   // the first apply reads p (times beta = 0)
     const T z = d ? d[t] * bv : bv;
     rho += (double)bv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
     mx = nanmax(mx, fabs((double)bv));
@@ -445,8 +446,8 @@ struct PcgRun {
   int init(const T* b, T* x) {
     hipStream_t st = ctx->stream;
     SEM_HIP(hipMemsetAsync(state, 0, sizeof(PcgState), st));
-    hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(256), 0, st, b, x, r, d, g.P, g.L, g.NC,
-                       g.per_x, g.per_y, n, part, n3);
+    hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(256), 0, st, b, x, r, p, d, g.P, g.L,
+                       g.NC, g.per_x, g.per_y, n, part, n3);
     SEM_HIP(hipGetLastError());
     return SEM_OK;
   }
@@ -697,6 +698,9 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
     scale = pow2_near(res);
     hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, st, r64, R.r, 1.0 / scale, n);
     SEM_HIP(hipMemsetAsync(R.state, 0, sizeof(PcgState), st));
+    // the first apply of a restarted cycle reads p times beta = 0: p must be
+    // finite (the old direction may not be, after a rolled-back cycle)
+    SEM_HIP(hipMemsetAsync(R.p, 0, sizeof(float) * n, st));
     k = 1;
     return R.reprime(1);
   };
