@@ -42,7 +42,10 @@ enum sem_status {
 /* Operator precision. The MFMA forms (nr == ns == 16) keep FP32 fields and
  * run the Dr/Ds contractions on matrix cores with FP16 or BF16 inputs and
  * FP32 accumulation; the X3 forms split each input into hi + lo and sum
- * three products (lo*hi + hi*lo + hi*hi), about FP32 accuracy. */
+ * three products (lo*hi + hi*lo + hi*hi), about FP32 accuracy. The
+ * single-pass forms round each contracted vector after removing its linear
+ * interpolant through the two end nodes (added back exactly in FP32), so
+ * smooth fields lose no more than the FP32 operator does. */
 enum sem_precision {
   SEM_F64 = 0,
   SEM_F32 = 1,
@@ -244,9 +247,9 @@ int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond, const voi
  * carry over (the Krylov space is not restarted); at each replacement the
  * inner vectors are rescaled by a power of two so the residual is O(1). A
  * cycle whose true residual is not finite or grew tenfold is rolled back and
- * the Krylov space restarted. FP32 and split (x3) operators keep the search
- * direction across replacements until one shows no progress, then restart it
- * at every replacement; single-pass FP16/BF16-input operators always restart
+ * the Krylov space restarted. FP32, FP16 and split (x3) operators keep the
+ * search direction across replacements until one shows no progress, then
+ * restart it at every replacement; the single-pass BF16 operator always restarts
  * it (classical defect correction).
  * SEM_MIXED_RELIABLE=1 / 0 forces one policy. Stops when norm(b - op64(x), Inf) <= tol;
  * SEM_WARN_MAXITER after max_outer replacements, four replacements without a

@@ -57,6 +57,8 @@ struct AxP {
   const T* __restrict__ B;
   const T* __restrict__ Dr;   // P x P column-major: Dr(i + P*j) = Dr_ij
   const T* __restrict__ Ds;
+  const double* __restrict__ zr;   // GLL nodes and weights (deflated MFMA forms)
+  const double* __restrict__ wr;
   const T* __restrict__ nu_f;
   const T* __restrict__ k_f;
   T nu, k;
@@ -164,7 +166,8 @@ __device__ __forceinline__ mf_f4 mf_contract(typename MfFrag<MF>::type ah,
 // MODE 2: one fused PCG iteration body (see AxP). MF selects the MFMA
 // contraction path (T = float, P = 16 only).
 template <typename T, int P, int CX, int MODE, int MF = 0>
-__global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
+__global__ void __launch_bounds__(CX* P)
+    __attribute__((amdgpu_waves_per_eu((MF == 1 || MF == 2) ? 3 : 1))) k_ax(const AxP<T> prm) {
   constexpr int NR = CX * P;
   constexpr int RS = NR + 1;
   static_assert(MF == 0 || (P == 16 && NR == 256), "MFMA path: 16x16 blocks");
@@ -190,13 +193,39 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
   Frag aD, aDl, aT, aTl;
   T aDk[4], aTk[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // Single-pass FP16/BF16 forms (MF 1, 2) deflate every contracted 1-D
+  // vector v by its linear interpolant l(z) = c0 + c1 z through the two end
+  // nodes before rounding it: A v = A (v - l) + c0 (A 1) + c1 (A z), the last
+  // two terms exact in FP32. For smooth fields v - l is O(h^2 v'') instead of
+  // O(v), which removes the rounding error of the constant and linear parts
+  // (the modes a low-precision operator corrupts most, relative to A v).
+  // Derivatives: A = D, D 1 = 0, D z = 1. Weak-form transposes: D' acts on
+  // fluxes that carry the GLL weight of the contracted index, so the
+  // operand is A = D' diag(w / wm) on f = flux .* (wm / w) (wm = max w keeps
+  // f in FP16 range), with A 1 = (e_N - e_0) / wm and
+  // A z = (e_N + e_0 - w) / wm (GLL quadrature is exact for both integrals).
+  constexpr bool kDefl = MF == 1 || MF == 2;
+  // deflation tables (nodes z_k, wm / w_k, (A z)_k, (A 1)_k), in LDS rather
+  // than registers: the PCG-mode kernel sits at the register limit of three
+  // waves per SIMD
+  __shared__ float4 sdefl[kDefl ? 16 : 1];
   if constexpr (MF >= 1 && MF <= 4) {
     const int m = lane & 15, kg = lane >> 4;
     float dv[4], tv[4];
+    double wm = 0.0;
+    if constexpr (kDefl)
+      for (int q = 0; q < 16; ++q) wm = fmax(wm, prm.wr[q]);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      dv[q] = (float)prm.Dr[m + 16 * (4 * kg + q)];
-      tv[q] = (float)prm.Dr[(4 * kg + q) + 16 * m];
+      const int k = 4 * kg + q;
+      dv[q] = (float)prm.Dr[m + 16 * k];
+      tv[q] = kDefl ? (float)(prm.Dr[k + 16 * m] * (prm.wr[k] / wm)) : (float)prm.Dr[k + 16 * m];
+    }
+    if (kDefl && threadIdx.x < 16) {
+      const int k = threadIdx.x;
+      const double eN = k == 15 ? 1.0 : 0.0, e0 = k == 0 ? 1.0 : 0.0;
+      sdefl[k] = make_float4((float)prm.zr[k], (float)(wm / prm.wr[k]),
+                             (float)((eN + e0 - prm.wr[k]) / wm), (float)((eN - e0) / wm));
     }
     mf_pack<MF>(dv, aD, aDl);
     mf_pack<MF>(tv, aT, aTl);
@@ -371,6 +400,23 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
             bu[q] = (float)s0[n * RS + e2 * 16 + 4 * kg + q];
             bt[q] = (float)s0[(4 * kg + q) * RS + e2 * 16 + n];
           }
+          float c1u = 0.f, c1t = 0.f;
+          if constexpr (kDefl) {
+            float dz[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dz[q] = sdefl[4 * kg + q].x;
+            // end nodes of column n (r-contraction) and of row n (s-contraction)
+            const float u0 = (float)s0[n * RS + e2 * 16], u1 = (float)s0[n * RS + e2 * 16 + 15];
+            const float t0 = (float)s0[e2 * 16 + n], t1 = (float)s0[15 * RS + e2 * 16 + n];
+            const float c0u = 0.5f * (u0 + u1), c0t = 0.5f * (t0 + t1);
+            c1u = 0.5f * (u1 - u0);
+            c1t = 0.5f * (t1 - t0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              bu[q] = bu[q] - c0u - c1u * dz[q];
+              bt[q] = bt[q] - c0t - c1t * dz[q];
+            }
+          }
           Frag bh, bl, th, tl;
           mf_pack<MF>(bu, bh, bl);
           mf_pack<MF>(bt, th, tl);
@@ -378,8 +424,8 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
           const mf_f4 us4 = mf_contract<MF>(aD, aDl, th, tl);
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            s1[n * RS + e2 * 16 + 4 * kg + q] = ur4[q];
-            s0[(4 * kg + q) * RS + e2 * 16 + n] = us4[q];
+            s1[n * RS + e2 * 16 + 4 * kg + q] = kDefl ? ur4[q] + c1u : ur4[q];
+            s0[(4 * kg + q) * RS + e2 * 16 + n] = kDefl ? us4[q] + c1t : us4[q];
           }
         } else {
           T bu[4], bt[4];
@@ -467,6 +513,26 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
             bw[q] = (float)s0[n * RS + e2 * 16 + 4 * kg + q];
             bs[q] = (float)s1[(4 * kg + q) * RS + e2 * 16 + n];
           }
+          float c0w = 0.f, c1w = 0.f, c0s = 0.f, c1s = 0.f;
+          float4 dq[4];
+          if constexpr (kDefl) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dq[q] = sdefl[4 * kg + q];
+            const float dwi0 = sdefl[0].y, dwi15 = sdefl[15].y;
+            const float w0 = (float)s0[n * RS + e2 * 16] * dwi0;
+            const float w1 = (float)s0[n * RS + e2 * 16 + 15] * dwi15;
+            const float v0 = (float)s1[e2 * 16 + n] * dwi0;
+            const float v1 = (float)s1[15 * RS + e2 * 16 + n] * dwi15;
+            c0w = 0.5f * (w0 + w1);
+            c1w = 0.5f * (w1 - w0);
+            c0s = 0.5f * (v0 + v1);
+            c1s = 0.5f * (v1 - v0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              bw[q] = bw[q] * dq[q].y - c0w - c1w * dq[q].x;
+              bs[q] = bs[q] * dq[q].y - c0s - c1s * dq[q].x;
+            }
+          }
           Frag wh, wl, sh, sl;
           mf_pack<MF>(bw, wh, wl);
           mf_pack<MF>(bs, sh, sl);
@@ -474,8 +540,10 @@ __global__ void __launch_bounds__(CX* P) k_ax(const AxP<T> prm) {
           const mf_f4 p2t = mf_contract<MF>(aT, aTl, sh, sl);
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            s1[n * RS + e2 * 16 + 4 * kg + q] = p1[q];
-            s0[(4 * kg + q) * RS + e2 * 16 + n] = p2t[q];
+            s1[n * RS + e2 * 16 + 4 * kg + q] =
+                kDefl ? p1[q] + (c0w * dq[q].w + c1w * dq[q].z) : p1[q];
+            s0[(4 * kg + q) * RS + e2 * 16 + n] =
+                kDefl ? p2t[q] + (c0s * dq[q].w + c1s * dq[q].z) : p2t[q];
           }
         } else {
           T bw[4], bs[4];
@@ -719,6 +787,8 @@ AxP<T> make_params(const Layout& g, const ApplyArgs& a, const AxPlan& pl) {
   prm.G22 = static_cast<const T*>(a.G22);
   prm.B = static_cast<const T*>(a.B);
   prm.Dr = static_cast<const T*>(a.Dr);
+  prm.zr = a.zr;
+  prm.wr = a.wr;
   prm.Ds = static_cast<const T*>(a.Ds);
   prm.nu_f = static_cast<const T*>(a.nu_f);
   prm.k_f = static_cast<const T*>(a.k_f);

@@ -656,12 +656,15 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
     return s;
   }
   const bool debug = std::getenv("SEM_DEBUG_MIXED") != nullptr;
-  // The FP32 and split (x3) operators keep the search direction across
-  // replacements; single-pass FP16/BF16 operators restart it.
+  // The FP32, FP16 and split (x3) operators keep the search direction across
+  // replacements; the BF16 operator restarts it (measured at 128x128
+  // elements, N = 15: kept, BF16 needs 28,000-30,000 inner iterations,
+  // restarted 21,000).
   // SEM_MIXED_RELIABLE=1 / 0 forces one policy.
   const char* menv = std::getenv("SEM_MIXED_RELIABLE");
   bool reliable = menv ? std::atoi(menv) != 0
-                       : (op_lo->precision == SEM_F32 || op_lo->precision == SEM_F16X3_MFMA ||
+                       : (op_lo->precision == SEM_F32 || op_lo->precision == SEM_F16_MFMA ||
+                          op_lo->precision == SEM_F16X3_MFMA ||
                           op_lo->precision == SEM_BF16X3_MFMA);
   hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;   // the context's timing events
   if (hipEventRecord(e0, st) != hipSuccess ||

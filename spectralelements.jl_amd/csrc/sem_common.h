@@ -131,6 +131,8 @@ struct ApplyArgs {
   const void* B;
   const void* Dr;
   const void* Ds;
+  const double* zr;   // GLL nodes and weights (the deflated MFMA forms read them)
+  const double* wr;
   const void* nu_f;
   const void* k_f;
   double nu, k;

@@ -163,6 +163,8 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   const bool f64 = op->precision == SEM_F64;
   a.Dr = f64 ? (const void*)m->Dr : (const void*)m->Dr32;
   a.Ds = f64 ? (const void*)m->Ds : (const void*)m->Ds32;
+  a.zr = m->zr;
+  a.wr = m->wr;
   a.nu_f = op->nu_f;
   a.k_f = op->k_f;
   a.nu = op->desc.nu;

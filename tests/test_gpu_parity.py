@@ -416,6 +416,52 @@ def test_mfma_operator_matches_oracle(se, rows_env, prec):
     assert _rel(ye, o.hlmz(u, 1.0, 1.0, mo)) < MFMA_TOL[prec]
 
 
+# relative error of one apply on the smoothest mode sin(pi (x+1)/2) sin(pi
+# (y+1)/2) at 32x32 elements, N = 15: the single-pass FP16 / BF16 forms deflate
+# each contracted vector by its end-node linear interpolant, which keeps this
+# error near the FP32 operator's instead of 3-4 orders above it (without the
+# deflation the BF16 form is off by a factor of about 100 on this mode).
+SMOOTH_TOL = {"f32": 1e-2, "f16": 6e-2, "bf16": 0.4}
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16"])
+def test_low_precision_operator_on_smooth_mode(se, prec):
+    _log(f"smooth-mode apply {prec}")
+    m, mo = _pair(se, 16, 32, (False, False), "sinsin")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    u = Mo * np.sin(np.pi * (mo.x + 1) / 2) * np.sin(np.pi * (mo.y + 1) / 2)
+    ref = o.opLHS(u, 1.0, 1.0, mo, Mo)
+    op = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+    y = se.to_host(op(se.to_device(u, dtype=op.dtype)).double())
+    assert _rel(y, ref) < SMOOTH_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16", "f16x3", "bf16x3"])
+def test_mixed_solve_full_size_reaches_fp64_tolerance(se, prec):
+    """128x128 elements, N = 15, deformed, Helmholtz, f ~ N(0,1) per global
+    node (seed 3), inner tolerance 1e-5: every variant reaches
+    norm(b - A64 x, Inf) <= 1e-10 norm(b, Inf) and agrees with the FP64 solve."""
+    import torch
+    from semhip import workloads as W
+    _log(f"mixed full size {prec}")
+    m = se.Mesh(16, 16, 128, 128, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    b = W.helmholtz_rhs(se, m, M, seed=3)
+    tol = 1e-10 * float(b.abs().max())
+    d = op64.jacobi()
+    op_lo = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+    info = se.PCGInfo()
+    x = se.pcg_mixed(b, op64, op_lo, d=d, tol=tol, inner_tol=1e-5, max_outer=200, info=info)
+    assert info.converged, (prec, info.outer, info.iters, info.res)
+    assert float((b - op64(x)).abs().max()) <= 1.01 * tol
+    x64 = se.pcg(b, op64, opM=se.MG(op64), mult=m.mult, tol=tol)
+    assert float((x - x64).abs().max()) <= 1e-8 * float(x64.abs().max())
+    del x, x64
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("prec", ["f32", "f16", "bf16", "f16x3", "bf16x3"])
 def test_mixed_solve_reaches_fp64_tolerance(se, prec):
     """FP64 defect correction around each low-precision inner solve reaches
