@@ -54,6 +54,13 @@ struct AxP {
   const T* __restrict__ G11;
   const T* __restrict__ G12;
   const T* __restrict__ G22;
+  // The single-pass FP16/BF16 operators store G11, G12, G22 and B as FP16,
+  // each scaled by a power of two (gs = the inverse scales, exact)
+  const _Float16* __restrict__ h11;
+  const _Float16* __restrict__ h12;
+  const _Float16* __restrict__ h22;
+  const _Float16* __restrict__ hB;
+  float gs[4];
   const T* __restrict__ B;
   const T* __restrict__ Dr;   // P x P column-major: Dr(i + P*j) = Dr_ij
   const T* __restrict__ Ds;
@@ -238,6 +245,25 @@ __global__ void __launch_bounds__(CX* P)
     }
   }
 
+  // geometric factors and mass in the operator's storage precision
+  constexpr bool kHalfG = MF == 1 || MF == 2;
+  auto ld_g11 = [&](int idx) -> T {
+    if constexpr (kHalfG) return T((float)prm.h11[idx] * prm.gs[0]);
+    else return prm.G11[idx];
+  };
+  auto ld_g12 = [&](int idx) -> T {
+    if constexpr (kHalfG) return T((float)prm.h12[idx] * prm.gs[1]);
+    else return prm.G12[idx];
+  };
+  auto ld_g22 = [&](int idx) -> T {
+    if constexpr (kHalfG) return T((float)prm.h22[idx] * prm.gs[2]);
+    else return prm.G22[idx];
+  };
+  auto ld_b = [&](int idx) -> T {
+    if constexpr (kHalfG) return T((float)prm.hB[idx] * prm.gs[3]);
+    else return prm.B[idx];
+  };
+
   const int tid = threadIdx.x;
   // PCG: the iterate is advanced here, one iteration late
   constexpr bool kXLate = MODE == 2;
@@ -327,9 +353,9 @@ __global__ void __launch_bounds__(CX* P)
       if (kEarlyG) {
 #pragma unroll
         for (int b = 0; b < (kEarlyG ? P : 1); ++b) {
-          g11[b] = prm.G11[bc_ + L * b];
-          g12[b] = prm.G12[bc_ + L * b];
-          g22[b] = prm.G22[bc_ + L * b];
+          g11[b] = ld_g11(bc_ + L * b);
+          g12[b] = ld_g12(bc_ + L * b);
+          g22[b] = ld_g22(bc_ + L * b);
         }
       }
     };
@@ -471,9 +497,9 @@ __global__ void __launch_bounds__(CX* P)
           x12 = g12[kEarlyG ? b : 0];
           x22 = g22[kEarlyG ? b : 0];
         } else {
-          x11 = prm.G11[bc_ + L * b];
-          x12 = prm.G12[bc_ + L * b];
-          x22 = prm.G22[bc_ + L * b];
+          x11 = ld_g11(bc_ + L * b);
+          x12 = ld_g12(bc_ + L * b);
+          x22 = ld_g22(bc_ + L * b);
         }
         const T usb = MF ? s0[b * RS + tid] : us[b];
         const T wr = x11 * ur + x12 * usb;
@@ -496,7 +522,12 @@ __global__ void __launch_bounds__(CX* P)
     // branch would make the compiler wait for every outstanding load at the
     // join, before the barrier, instead of letting B's latency hide under C2
     T bv[P];
-    {
+    if constexpr (kHalfG) {
+      const _Float16* bsrc = (prm.has_mass && plain) ? prm.hB : prm.h22;
+      const float sc = (prm.has_mass && plain) ? prm.gs[3] : prm.gs[2];
+#pragma unroll
+      for (int b = 0; b < P; ++b) bv[b] = T((float)bsrc[bc_ + L * b] * sc);
+    } else {
       const T* bsrc = (prm.has_mass && plain) ? prm.B : prm.G22;
 #pragma unroll
       for (int b = 0; b < P; ++b) bv[b] = bsrc[bc_ + L * b];
@@ -598,7 +629,7 @@ __global__ void __launch_bounds__(CX* P)
         T hv = nuv * au;
         if (prm.has_mass) {
           const T kv = prm.k_f ? prm.k_f[idx] : prm.k;
-          hv = hv + kv * (prm.B[idx] * uv[b]);
+          hv = hv + kv * (ld_b(idx) * uv[b]);
         }
         h[b] = hv;
       }
@@ -786,6 +817,11 @@ AxP<T> make_params(const Layout& g, const ApplyArgs& a, const AxPlan& pl) {
   prm.G12 = static_cast<const T*>(a.G12);
   prm.G22 = static_cast<const T*>(a.G22);
   prm.B = static_cast<const T*>(a.B);
+  prm.h11 = static_cast<const _Float16*>(a.G11);
+  prm.h12 = static_cast<const _Float16*>(a.G12);
+  prm.h22 = static_cast<const _Float16*>(a.G22);
+  prm.hB = static_cast<const _Float16*>(a.B);
+  for (int q = 0; q < 4; ++q) prm.gs[q] = a.gscale[q];
   prm.Dr = static_cast<const T*>(a.Dr);
   prm.zr = a.zr;
   prm.wr = a.wr;

@@ -164,6 +164,11 @@ __global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = (float)x[t];
 }
+__global__ void k_f64_to_f16(const double* __restrict__ x, double scale,
+                             _Float16* __restrict__ y, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (_Float16)(float)(x[t] * scale);
+}
 __global__ void k_f32_to_f64(const float* __restrict__ x, double* __restrict__ y, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = (double)x[t];
@@ -651,6 +656,12 @@ int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float
 
 int launch_convert_f64_f32(hipStream_t st, int64_t n, const double* x, float* y) {
   hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n)), dim3(256), 0, st, x, y, (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+int launch_convert_f64_f16(hipStream_t st, int64_t n, const double* x, double scale, void* y) {
+  hipLaunchKernelGGL(k_f64_to_f16, dim3(nblk(n)), dim3(256), 0, st, x, scale,
+                     static_cast<_Float16*>(y), (long)n);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

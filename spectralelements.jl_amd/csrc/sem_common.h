@@ -105,6 +105,10 @@ struct sem_operator {
   bool has_mass = false;
   // MFMA operand tables (FP16/BF16 bit patterns) for the contraction matrices
   void* Dmfma = nullptr;
+  // single-pass FP16/BF16 operators: G11, G12, G22, B stored as FP16 scaled
+  // by 2^e (e chosen per array so the largest magnitude is near 2^14);
+  // gscale = 2^-e
+  float gscale[4] = {1.f, 1.f, 1.f, 1.f};
   // iterations of the last fresh solve with this operator: the first launch
   // batch of the next one (repeated solves: time steps, adjoints, FD)
   mutable int last_iters = 0;
@@ -140,6 +144,8 @@ struct ApplyArgs {
   int bc;
   int assemble;
   int mfma;  // 0: FMA contractions; 1..4: MFMA (FP16, BF16, FP16 x3, BF16 x3)
+  // mfma 1, 2: G11, G12, G22, B are FP16 arrays; value = stored * gscale[q]
+  float gscale[4];
 };
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);
@@ -219,6 +225,8 @@ int launch_mul(hipStream_t st, int64_t n, const double* a, const double* u, doub
 int launch_abu(hipStream_t st, int m, int n, const double* u, const double* As, int ma,
                int na, const double* Br, int mb, int nb, int trans, double* tmp, double* y);
 int launch_convert_f64_f32(hipStream_t st, int64_t n, const double* x, float* y);
+// y = (FP16)(x * scale)
+int launch_convert_f64_f16(hipStream_t st, int64_t n, const double* x, double scale, void* y);
 int launch_convert_f32_f64(hipStream_t st, int64_t n, const float* x, double* y);
 int launch_jacobi_local(hipStream_t st, const sem_operator* op, double* dl);
 int launch_inv_masked(hipStream_t st, const Layout& g, int bits, const double* d, double* dinv);

@@ -173,6 +173,7 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   a.bc = op->desc.assemble ? op->desc.bc_bits : 0;
   a.assemble = op->desc.assemble;
   a.mfma = mfma_mode(op->precision);
+  for (int q = 0; q < 4; ++q) a.gscale[q] = op->gscale[q];
   return a;
 }
 
@@ -551,10 +552,47 @@ int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc
                           desc->G22 ? desc->G22 : m->f[SEM_G22],
                           desc->B ? desc->B : m->f[SEM_B], desc->nu_f, desc->k_f};
   void** dst[6] = {&op->G11, &op->G12, &op->G22, &op->B, &op->nu_f, &op->k_f};
+  const bool half = precision == SEM_F16_MFMA || precision == SEM_BF16_MFMA;
   for (int q = 0; q < 6; ++q) {
     if (!src[q]) continue;
     if (precision == SEM_F64) {
       *dst[q] = const_cast<double*>(src[q]);
+    } else if (half && q < 4) {
+      // FP16 storage, scaled by 2^e so that the largest magnitude is near 2^14
+      int s = ensure_ws(ctx, sizeof(double) * 257);
+      double mx = 0.0;
+      if (s == SEM_OK) {
+        double* part = static_cast<double*>(ctx->ws);
+        s = launch_norm_inf(ctx->stream, m->nl, src[q], part, 256, part + 256);
+        if (s == SEM_OK && (hipMemcpyAsync(&mx, part + 256, sizeof(double), hipMemcpyDeviceToHost,
+                                           ctx->stream) != hipSuccess ||
+                            hipStreamSynchronize(ctx->stream) != hipSuccess)) {
+          set_error("sem_operator_create: reading the factor range failed");
+          s = SEM_ERR_HIP;
+        }
+      }
+      if (s == SEM_OK && !std::isfinite(mx)) {
+        set_error("sem_operator_create: non-finite geometric factor");
+        s = SEM_ERR_NONFINITE;
+      }
+      void* h = nullptr;
+      if (s == SEM_OK && hipMalloc(&h, sizeof(uint16_t) * m->nl) != hipSuccess) {
+        set_error("sem_operator_create: hipMalloc failed");
+        s = SEM_ERR_HIP;
+      }
+      if (s != SEM_OK) {
+        sem_operator_destroy(op);
+        return s;
+      }
+      op->owned[q] = true;
+      *dst[q] = h;
+      const int e = mx > 0.0 ? 14 - std::ilogb(mx) : 0;
+      op->gscale[q] = std::ldexp(1.0f, -e);
+      s = launch_convert_f64_f16(ctx->stream, m->nl, src[q], std::ldexp(1.0, e), h);
+      if (s != SEM_OK) {
+        sem_operator_destroy(op);
+        return s;
+      }
     } else {
       float* f = nullptr;
       if (hipMalloc(&f, sizeof(float) * m->nl) != hipSuccess) {
