@@ -55,9 +55,6 @@ k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __r
   }
 }
 
-// rows whose facts are tabulated in LDS (64 KiB); longer rows recompute them
-constexpr int kRowTable = 16384;
-
 // x-direction facts of local row i, packed: bit 0 = half multiplicity in x
 // (mult1), bits 1-2 = seam kind of the strip kernel's unsummed copies
 // (0 none, 1 pair (i-1, i), 2 pair (i, i+1), 3 periodic wrap (L-1, 0)),
@@ -76,11 +73,48 @@ __device__ inline int row_info(int i, const SeamGeom& g) {
   return v;
 }
 
+// y-direction facts of local column j: x = bit 0 half multiplicity in y,
+// bit 1 masked by a y Dirichlet side, bit 2 a segment y-seam column (the
+// strip kernel left its copies and those of column jR unsummed), bit 3 the
+// seam pair is the periodic wrap (NC-1, 0); y = jL (jR = jL + 1 unless wrap)
+__device__ inline int2 col_info(int j, const SeamGeom& g) {
+  int f = 0, jl = 0;
+  const int b = j % g.P;
+  if ((b == 0 && (j > 0 || g.per_y)) || (b == g.P - 1 && (j < g.NC - 1 || g.per_y))) f |= 1;
+  if (((g.bc & SEM_BC_YMIN) && j == 0) || ((g.bc & SEM_BC_YMAX) && j == g.NC - 1)) f |= 2;
+  const int qy = j % g.SP;
+  if (qy == 0 && j > 0) {
+    f |= 4;
+    jl = j - 1;
+  } else if (qy == g.SP - 1 && j < g.NC - 1) {
+    f |= 4;
+    jl = j;
+  } else if (g.per_y && (j == 0 || j == g.NC - 1)) {
+    f |= 4 | 8;
+    jl = g.NC - 1;
+  }
+  return make_int2(f, jl);
+}
+
+// Row and column fact tables of one solve (written once, before the
+// iterations): the update kernel reads them instead of recomputing integer
+// remainders per node (which made it scalar-instruction bound).
+__global__ void k_pcg_tables(int* __restrict__ rinfo, int2* __restrict__ cinfo, SeamGeom g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < g.L) rinfo[t] = row_info(t, g);
+  if (t < g.NC) cinfo[t] = col_info(t, g);
+}
+
+// rows whose fact table is copied into LDS (64 KiB); longer rows read it
+// from global memory
+constexpr int kRowTable = 16384;
+
 // One node of the PCG update: r -= alpha wf, partials of <r, z>_mult (m =
 // the node's mult) and norm(r, Inf). With keep == false the node is left as
-// it is (its old r is stored back) and adds nothing to the partials: a select, not a branch, so the bulk pass has no divergent
-// control flow. The max ignores NaN here; the block restores NaN from the
-// <r, z> partial, which is NaN whenever some r is.
+// it is (its old r is stored back) and adds nothing to the partials: a
+// select, not a branch, so the bulk pass has no divergent control flow. The
+// max ignores NaN here; the block restores NaN from the <r, z> partial, which
+// is NaN whenever some r is.
 // The iterate is not touched here: the next iteration's apply adds alpha p
 // while it reads p anyway, and k_pcg_xlast adds the last one.
 template <typename T, bool HAS_D>
@@ -101,26 +135,30 @@ __device__ __forceinline__ void upd_node(T* __restrict__ r, const T* __restrict_
 // partials of <r, z>_mult and norm(r, Inf) for the next iteration. A block
 // walks whole columns j (its share of NC). Bulk pass: each thread takes rows
 // tid, tid + kUpdThreads, ... except the strip-seam rows; the per-row facts
-// come from an LDS table built once per block, the per-column facts (segment
-// y-seam, mask, mult) are uniform, so the bulk has no per-node integer
+// come from the row table, the per-column facts (segment y-seam, mask, mult)
+// from the column table in one uniform load, so the bulk has no integer
 // division and no divergent seam branches. Seam pass: the 2 (nstrips - 1)
 // strip-seam rows (+ 2 periodic wrap rows) of the block's columns, with the
 // four-copy sums at element y-interfaces.
 constexpr int kUpdThreads = 1024;   // 16 waves per block: latency hiding for FP32
+// most update blocks (and block partials of <r, z>, norm(r, Inf)) per solve
+#ifndef SEM_PCG_NPART
+#define SEM_PCG_NPART 256
+#endif
 
-// HAS_D: a diagonal preconditioner is present. FULL: L is a multiple of
-// kUpdThreads and the row table fits, so the bulk row loop has a uniform trip
-// count (scalar loop control, no per-lane exit masks).
+// HAS_D: a diagonal preconditioner is present. VEC = 2: two adjacent rows per
+// thread with 8-byte accesses (FP32, L even). FULL: L is a multiple of
+// VEC * kUpdThreads, so the row loop needs no bound test.
 template <typename T, bool HAS_D, bool FULL, int VEC = 1>
 __global__ void __launch_bounds__(kUpdThreads)
-k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d, const double* __restrict__ Q,
-             int nQ, PcgState* __restrict__ st, int k, double* __restrict__ part, int n3,
-             SeamGeom sg, long n) {
-  const int P = sg.P, NC = sg.NC, per_y = sg.per_y;
+k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d,
+             const double* __restrict__ Q, int nQ, PcgState* __restrict__ st, int k,
+             double* __restrict__ part, int n3, SeamGeom sg, const int* __restrict__ rtab,
+             const int2* __restrict__ ctab, long n) {
+  const int NC = sg.NC;
   const int Lr = sg.L;
   const long L = sg.L;
   __shared__ double red[kUpdThreads];
-  extern __shared__ int rinfo[];
   // independent loads first, so their latencies overlap: this thread's share
   // of the <p, Ap> partials and t = <r, z> of this iteration (both valid
   // memory even when the solve has already stopped)
@@ -130,11 +168,17 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
   // done is written only by the apply launches, so it cannot change while
   // this kernel runs: every thread reads the same value, no broadcast needed
   if (st->done) return;
-  const bool table = FULL || Lr <= kRowTable;   // uniform
-  if (table)
-    for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rinfo[i] = row_info(i, sg);
-  // synchronises the block; red is next written by the final reduction,
-  // after the barrier that precedes the seam pass
+  // one-row-per-thread passes read the row table from LDS when it fits (an
+  // LDS read per node is cheaper than a global one there; the paired-row FP32
+  // pass reads one 8-byte global entry per pair, which measured faster);
+  // either way the table is the solve's, so no integer remainders here
+  extern __shared__ int rlds[];
+  const bool in_lds = VEC == 1 && Lr <= kRowTable;   // uniform
+  if (in_lds)
+    for (int i = threadIdx.x; i < Lr; i += kUpdThreads) rlds[i] = rtab[i];
+  const int* __restrict__ rinfo = in_lds ? rlds : rtab;
+  // synchronises the block (also for the table); red is next written by the
+  // final reduction, after the barrier that precedes the seam pass
   const double pw = block_sum_w_nb<kUpdThreads>(qs, red);
   const T alpha = T(tk / pw);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = (double)alpha;
@@ -144,32 +188,20 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
   double rho = 0.0, mx = 0.0;
   // bulk: every row but the strip-seam rows
   for (int j = j0; j < j1; ++j) {
-    const int b = j % P;
-    const bool yhalf = (b == 0 && (j > 0 || per_y)) || (b == P - 1 && (j < NC - 1 || per_y));
-    const double my = yhalf ? 0.5 : 1.0;
-    const int qy = j % sg.SP;   // segment y-seam
-    int jL = -1, jR = -1;
-    if (qy == 0 && j > 0) {
-      jL = j - 1;
-      jR = j;
-    } else if (qy == sg.SP - 1 && j < NC - 1) {
-      jL = j;
-      jR = j + 1;
-    } else if (per_y && (j == 0 || j == NC - 1)) {
-      jL = NC - 1;
-      jR = 0;
-    }
-    const bool masky = ((sg.bc & SEM_BC_YMIN) && j == 0) || ((sg.bc & SEM_BC_YMAX) && j == NC - 1);
+    const int2 ci = ctab[j];   // uniform
+    const double my = (ci.x & 1) ? 0.5 : 1.0;
+    const bool masky = (ci.x & 2) != 0;
+    const bool yseam = (ci.x & 4) != 0;
+    const int jL = ci.y, jR = (ci.x & 8) ? 0 : ci.y + 1;
     const long cj = L * j;
-    if constexpr (FULL && VEC == 2) {
-      // two adjacent rows per thread with 8-byte accesses (FP32)
+    if constexpr (VEC == 2) {
       static_assert(sizeof(T) == 4, "paired rows: FP32");
       using T2 = float2;
-      const bool yseam = jL >= 0;
       const long cL = L * (yseam ? jL : 0), cR = L * (yseam ? jR : 0);
 #pragma unroll 2
       for (int i0 = 0; i0 < Lr; i0 += 2 * kUpdThreads) {
         const int i = i0 + 2 * (int)threadIdx.x;
+        if (!FULL && i >= Lr) break;
         const long t = i + cj;
         const int2 info = *reinterpret_cast<const int2*>(rinfo + i);
         T2 wf;
@@ -203,43 +235,27 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
         mx = fmax(mx, k0 ? fabs((double)rn.x) : 0.0);
         mx = fmax(mx, k1 ? fabs((double)rn.y) : 0.0);
       }
-    } else if constexpr (FULL) {
-      if (jL >= 0) {   // uniform: a segment y-seam column
-        const long cL = L * jL, cR = L * jR;
-#pragma unroll 2
-        for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
-          const int i = i0 + (int)threadIdx.x;
-          const int info = rinfo[i];
-          const T mv = ((info & 8) || masky) ? T(0) : T(1);
-          const T wf = (w[i + cL] + w[i + cR]) * mv;
-          upd_node<T, HAS_D>(r, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
-                             !(info & 6), rho, mx);
-        }
-      } else {
-#pragma unroll 2
-        for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
-          const int i = i0 + (int)threadIdx.x;
-          const int info = rinfo[i];
-          const long t = i + cj;
-          upd_node<T, HAS_D>(r, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
-                             !(info & 6), rho, mx);
-        }
-      }
-    } else if (jL >= 0) {
+    } else if (yseam) {   // uniform: a segment y-seam column
       const long cL = L * jL, cR = L * jR;
-      for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
-        const int info = table ? rinfo[i] : row_info(i, sg);
+#pragma unroll 2
+      for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
+        const int i = i0 + (int)threadIdx.x;
+        if (!FULL && i >= Lr) break;
+        const int info = rinfo[i];
         const T mv = ((info & 8) || masky) ? T(0) : T(1);
         const T wf = (w[i + cL] + w[i + cR]) * mv;
-        upd_node<T, HAS_D>(r, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my,
-                           !(info & 6), rho, mx);
+        upd_node<T, HAS_D>(r, d, i + cj, wf, alpha, (info & 1 ? 0.5 : 1.0) * my, !(info & 6),
+                           rho, mx);
       }
     } else {
-      for (int i = threadIdx.x; i < Lr; i += kUpdThreads) {
-        const int info = table ? rinfo[i] : row_info(i, sg);
+#pragma unroll 2
+      for (int i0 = 0; i0 < Lr; i0 += kUpdThreads) {
+        const int i = i0 + (int)threadIdx.x;
+        if (!FULL && i >= Lr) break;
+        const int info = rinfo[i];
         const long t = i + cj;
-        upd_node<T, HAS_D>(r, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my,
-                           !(info & 6), rho, mx);
+        upd_node<T, HAS_D>(r, d, t, w[t], alpha, (info & 1 ? 0.5 : 1.0) * my, !(info & 6), rho,
+                           mx);
       }
     }
   }
@@ -263,7 +279,8 @@ k_pcg_update(T* __restrict__ r, const T* __restrict__ w, const T* __restrict__ d
       iL = Lr - 1;
       iR = 0;
     }
-    const int info = row_info(i, sg);
+    const int info = rinfo[i];
+    const int P = sg.P, per_y = sg.per_y;
     const int b = j % P;
     const bool yhalf = (b == 0 && (j > 0 || per_y)) || (b == P - 1 && (j < NC - 1 || per_y));
     int jl = -1, ju = -1;   // element y-interface: four copies
@@ -393,6 +410,8 @@ struct PcgRun {
   ApplyArgs aa{};
   T *r = nullptr, *p = nullptr, *w = nullptr;
   double *part = nullptr, *Q = nullptr;
+  int* rtab = nullptr;
+  int2* ctab = nullptr;
   PcgState* state = nullptr;
   PcgState* hs = nullptr;
 
@@ -405,31 +424,41 @@ struct PcgRun {
     g = m->layout();
     n = (long)m->nl;
     pl = make_plan(g, true, true);
-    n3 = (int)std::min<long>(512, std::max<long>(1, (n + 2047) / 2048));
+    n3 = (int)std::min<long>(SEM_PCG_NPART, std::max<long>(1, (n + 2047) / 2048));
     nQ = pl.nk1;
     sg = SeamGeom{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x,
                   g.per_y, op->desc.bc_bits};
-    return 3 * vec_bytes() + part_bytes() + q_bytes() + 256;
+    return 3 * vec_bytes() + part_bytes() + q_bytes() + tab_bytes() + 256;
+  }
+  size_t tab_bytes() const {
+    return ((sizeof(int) * g.L + 255) / 256) * 256 + ((sizeof(int2) * g.NC + 255) / 256) * 256;
   }
   size_t vec_bytes() const { return ((sizeof(T) * n + 255) / 256) * 256; }
   size_t part_bytes() const { return sizeof(double) * 4 * n3; }
   size_t q_bytes() const { return ((sizeof(double) * nQ + 255) / 256) * 256; }
 
-  // binds the workspace (r, p, w | part[2][2*n3] | Q[nQ] | state) and the
-  // host copy of the state
-  void bind(void* ws, PcgState* host_state) {
-    const size_t vb = vec_bytes(), pb = part_bytes(), qb = q_bytes();
+  // binds the workspace (r, p, w | part[2][2*n3] | Q[nQ] | row table |
+  // column table | state) and the host copy of the state, and writes the
+  // row and column fact tables of the update kernel
+  int bind(void* ws, PcgState* host_state) {
+    const size_t vb = vec_bytes(), pb = part_bytes(), qb = q_bytes(), tb = tab_bytes();
     char* w0 = static_cast<char*>(ws);
     r = reinterpret_cast<T*>(w0);
     p = reinterpret_cast<T*>(w0 + vb);
     w = reinterpret_cast<T*>(w0 + 2 * vb);
     part = reinterpret_cast<double*>(w0 + 3 * vb);
     Q = reinterpret_cast<double*>(w0 + 3 * vb + pb);
-    state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb);
+    rtab = reinterpret_cast<int*>(w0 + 3 * vb + pb + qb);
+    ctab = reinterpret_cast<int2*>(w0 + 3 * vb + pb + qb + ((sizeof(int) * g.L + 255) / 256) * 256);
+    state = reinterpret_cast<PcgState*>(w0 + 3 * vb + pb + qb + tb);
     hs = host_state;
     aa = args_of(op, nullptr, w);   // factors, precision, recomputed geometry
     aa.bc = op->desc.bc_bits;
     aa.assemble = 1;
+    hipLaunchKernelGGL(k_pcg_tables, dim3(nblk(std::max(g.L, g.NC))), dim3(256), 0, ctx->stream,
+                       rtab, ctab, sg);
+    SEM_HIP(hipGetLastError());
+    return SEM_OK;
   }
 
   // workspace from the context (grown on demand), state read back through
@@ -438,8 +467,7 @@ struct PcgRun {
     const size_t bytes = plan(c, o, dd);
     SEM_TRY(ensure_ws(ctx, bytes));
     SEM_TRY(ensure_pinned(ctx, 128));  // PcgState, then a scalar slot at byte 64
-    bind(ctx->ws, static_cast<PcgState*>(ctx->pinned));
-    return SEM_OK;
+    return bind(ctx->ws, static_cast<PcgState*>(ctx->pinned));
   }
 
   // x = 0, r = b, partials of r for iteration 1, state cleared
@@ -463,30 +491,25 @@ struct PcgRun {
   }
 
   void launch_update(hipStream_t st, int k) {
-    const size_t lds = g.L <= kRowTable ? sizeof(int) * g.L : 0;
-    const bool full = g.L % kUpdThreads == 0 && g.L <= kRowTable;
-    // FP32 with rows in pairs: L a multiple of 2 kUpdThreads (8-byte aligned
-    // pairs, since every column starts at an even offset)
-    const bool pairs = sizeof(T) == 4 && g.L % (2 * kUpdThreads) == 0 && g.L <= kRowTable;
+    // FP32 with rows in pairs: L even (8-byte aligned pairs, since every
+    // column then starts at an even offset)
+    const bool pairs = sizeof(T) == 4 && g.L % 2 == 0;
+    const bool full = g.L % ((pairs ? 2 : 1) * kUpdThreads) == 0;
     double* pk = part + 2 * n3 * (k & 1);
+    const size_t lds = (!pairs && g.L <= kRowTable) ? sizeof(int) * g.L : 0;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, r, w, d, Q, nQ, state,
-                         k, pk, n3, sg, n);
+      hipLaunchKernelGGL(kern, dim3(n3), dim3(kUpdThreads), lds, st, r, w, d, Q, nQ, state, k,
+                         pk, n3, sg, rtab, ctab, n);
     };
     if constexpr (sizeof(T) == 4) {
       if (pairs) {
-        if (d) go(k_pcg_update<T, true, true, 2>);
-        else go(k_pcg_update<T, false, true, 2>);
+        if (d) full ? go(k_pcg_update<T, true, true, 2>) : go(k_pcg_update<T, true, false, 2>);
+        else full ? go(k_pcg_update<T, false, true, 2>) : go(k_pcg_update<T, false, false, 2>);
         return;
       }
     }
-    if (d) {
-      if (full) go(k_pcg_update<T, true, true>);
-      else go(k_pcg_update<T, true, false>);
-    } else {
-      if (full) go(k_pcg_update<T, false, true>);
-      else go(k_pcg_update<T, false, false>);
-    }
+    if (d) full ? go(k_pcg_update<T, true, true>) : go(k_pcg_update<T, true, false>);
+    else full ? go(k_pcg_update<T, false, true>) : go(k_pcg_update<T, false, false>);
   }
 
   // launches iterations k_start.. until the stop test fires (norm(r, Inf) <=
@@ -854,7 +877,7 @@ int pcg_noalloc_t(sem_ctx* ctx, const sem_operator* op, const T* d, const T* b, 
     return SEM_ERR_UNSUPPORTED;
   }
   PcgState hstate{};   // read back by a plain copy: no pinned buffer
-  R.bind(ws, &hstate);
+  SEM_TRY(R.bind(ws, &hstate));
   return pcg_solve(R, ctx, op, b, x, tol, maxiter, info);
 }
 
