@@ -306,6 +306,41 @@ int sem_mg_vcycle(sem_ctx* ctx, sem_mg* mg, const double* r, double* z);
 int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b, double* x,
                double tol, int maxiter, sem_pcg_info* info);
 
+/* ------------------------------------ mesh generation, point evaluation */
+/* gordonHall(xrm,xrp,xsm,xsp,yrm,yrp,ysm,ysp,zr,zs), geom.jl:8-31:
+ * transfinite interpolation of four boundary curves onto the nr x ns grid of
+ * the points r (nr) x s (ns) in [-1,1]: x = (1-r)/2 xrm(s) + (1+r)/2 xrp(s)
+ * + (1-s)/2 xsm(r) + (1+s)/2 xsp(r) - the bilinear interpolant of the
+ * corners (taken from the r-edges). xrm, xrp, yrm, yrp have ns entries (the
+ * r = -1 and r = +1 edges, indexed by s), xsm, xsp, ysm, ysp nr entries.
+ * All arrays are device arrays; x, y are nr x ns column-major. With r, s the
+ * 1-D node coordinates of a whole mesh (semmesh) this maps the mesh. The
+ * reference's corner term transposes the corner matrix (geom.jl:16-23), which
+ * differs from the transfinite interpolant unless x(1,-1) = x(-1,1); this
+ * is the interpolant the construction intends. */
+int sem_gordon_hall(sem_ctx* ctx, int nr, int ns, const double* r, const double* s,
+                    const double* xrm, const double* xrp, const double* xsm, const double* xsp,
+                    const double* yrm, const double* yrp, const double* ysm, const double* ysp,
+                    double* x, double* y);
+/* semInterp(u, xp, yp, rxe, rye, sxe, sye, zr, zs, xe, ye, tol, maxiter),
+ * interp.jl:47-102, for npts points at once over the Ex x Ey elements of an
+ * (nr x ns)-node layout: each point is located by scanning the elements
+ * whose bounding box holds it and running Newton on the element map from the
+ * closest GLL node (r += rx dx + ry dy, s += sx dx + sy dy with the inverse
+ * Jacobian interpolated at (r, s); the reference's update reads dx twice and
+ * dy twice, interp.jl:80-81) until |(dx, dy)| < tol within maxiter steps
+ * and inside [-1,1]^2; out = u interpolated there, NaN if no element
+ * converges; elem (optional) = the element (ex + Ex ey) or -1. Fields are
+ * device arrays in the local layout; xp, yp, out, elem are device arrays. */
+int sem_interp_points(sem_ctx* ctx, int nr, int ns, int Ex, int Ey, const double* u,
+                      const double* x, const double* y, const double* rx, const double* ry,
+                      const double* sx, const double* sy, int npts, const double* xp,
+                      const double* yp, double tol, int maxiter, double* out, int* elem);
+/* the same over a mesh (its x, y, rx, ry, sx, sy) */
+int sem_mesh_interp(sem_ctx* ctx, const sem_mesh* mesh, const double* u, int npts,
+                    const double* xp, const double* yp, double tol, int maxiter, double* out,
+                    int* elem);
+
 /* ----------------------------------------------------- vector utilities */
 /* sum(a .* b .* mult) with the mesh's mult (pcg.jl:45); result to host */
 int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* b,

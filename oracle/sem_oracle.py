@@ -38,6 +38,7 @@ __all__ = [
     "advect_transpose", "vardiff_operator", "vardiff_gradient", "cd_forward",
     "laplace_dealiased", "lapl12", "mass8",
     "cd_adjoint", "cd_objective", "annulus", "diffusion_simulate",
+    "gordonHall", "semInterp",
 ]
 
 
@@ -886,3 +887,68 @@ def diffusion_simulate(msh: Mesh, bc, Ti=0.0, Tf=0.0, dt=0.0, k=3, setIC=None, s
             if ts.time[0] < 1e-12:
                 break
     return u, iters
+
+
+# --------------------------------------------------------------------------
+# mesh generation and point evaluation (geom.jl, interp.jl)
+# --------------------------------------------------------------------------
+def gordonHall(xrm, xrp, xsm, xsp, yrm, yrp, ysm, ysp, zr, zs):
+    """Transfinite interpolation of four boundary curves (geom.jl:8-31):
+    x = Jer [xrm'; xrp'] + [xsm xsp] Jes' - Jer C Jes' with Jer =
+    interpMat(zr, [-1, 1]), Jes = interpMat(zs, [-1, 1]) and C the corner
+    values C[a, b] = x(r = ze[a], s = ze[b]), taken from the r-edges. (The
+    reference lays C out as C[b, a] (geom.jl:16-23), the transpose; this is
+    the interpolant the construction intends, and the two agree whenever
+    x(1, -1) = x(-1, 1).) Pinned in tests/test_oracle_geom.py by exact
+    reproduction of bilinear maps, interpolation of the edge data and a
+    closed-form Coons patch."""
+    ze = np.array([-1.0, 1.0])
+    Jer = interpMat(zr, ze)
+    Jes = interpMat(zs, ze)
+    out = []
+    for rm, rp, sm, sp in ((xrm, xrp, xsm, xsp), (yrm, yrp, ysm, ysp)):
+        rm, rp, sm, sp = (np.asarray(v, dtype=np.float64).reshape(-1) for v in (rm, rp, sm, sp))
+        C = np.array([[rm[0], rm[-1]], [rp[0], rp[-1]]])
+        er = Jer @ np.vstack([rm, rp])
+        es = np.column_stack([sm, sp]) @ Jes.T
+        out.append(er + es - Jer @ C @ Jes.T)
+    return out[0], out[1]
+
+
+def semInterp(u, xp, yp, rxe, rye, sxe, sye, zr, zs, xe, ye, tol=1e-8, maxiter=50):
+    """Value of the element field u at the physical point (xp, yp)
+    (interp.jl:47-102): Newton on the element map from the closest GLL node,
+    r += rx dx + ry dy, s += sx dx + sy dy with the inverse Jacobian
+    interpolated at (r, s), until |(dx, dy)| < tol. (The reference's update
+    reads rx dx + ry dx and sx dy + sy dy, interp.jl:80-81, which is not
+    Newton's step for a general map; this is the step it intends.) Returns
+    NaN when a step leaves [-1, 1]^2; after maxiter steps it returns the value
+    at the last iterate, as the reference does. Pinned in
+    tests/test_oracle_geom.py by recovering known (r, s) points of deformed
+    elements and exact values on affine elements."""
+    d2 = (xe - xp) ** 2 + (ye - yp) ** 2
+    ia, ib = np.unravel_index(np.argmin(d2), d2.shape)
+    r, s = zr[ia], zs[ib]
+    i = 0
+    while True:
+        Jr = interpMat(r, zr)
+        Js = interpMat(s, zs)
+        if i == maxiter:
+            break
+        x = float(ABu(Js, Jr, xe)[0, 0])
+        y = float(ABu(Js, Jr, ye)[0, 0])
+        dx, dy = xp - x, yp - y
+        if math.sqrt(dx * dx + dy * dy) < tol:
+            break
+        rx = float(ABu(Js, Jr, rxe)[0, 0])
+        ry = float(ABu(Js, Jr, rye)[0, 0])
+        sx = float(ABu(Js, Jr, sxe)[0, 0])
+        sy = float(ABu(Js, Jr, sye)[0, 0])
+        dr = rx * dx + ry * dy
+        ds = sx * dx + sy * dy
+        if abs(r + dr) > 1.0 or abs(s + ds) > 1.0:
+            return float("nan")
+        r += dr
+        s += ds
+        i += 1
+    return float(ABu(Js, Jr, u)[0, 0])

@@ -40,11 +40,11 @@ int dispatch(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunc
 
 }  // namespace ax
 
-AxPlan make_plan(const Layout& g, bool assemble, bool pcg) {
+AxPlan make_plan(const Layout& g, bool assemble, bool pcg, int rows) {
   AxPlan pl{};
   pl.CX = SEM_AX_THREADS / g.P;
   pl.nstrips = (g.Ex + pl.CX - 1) / pl.CX;
-  pl.S = assemble ? ax::rows_per_segment(g, pl.CX, pcg) : 1;
+  pl.S = assemble ? ax::rows_per_segment(g, pl.CX, pcg, rows) : 1;
   pl.nsegs = (g.Ey + pl.S - 1) / pl.S;
   pl.nk1 = pl.nstrips * pl.nsegs;
   if (assemble) {

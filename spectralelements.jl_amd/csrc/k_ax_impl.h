@@ -787,11 +787,10 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
   }
 }
 
-// Element rows swept per workgroup. SEM_AX_ROWS overrides the heuristic (used
-// by the tests to exercise every seam configuration).
-inline int rows_per_segment(const Layout& g, int cx, bool pcg) {
-  const char* env = getenv("SEM_AX_ROWS");
-  const int forced = env ? atoi(env) : 0;
+// Element rows swept per workgroup; forced > 0 overrides the heuristic (the
+// SEM_AX_ROWS hook, read when an operator is created, lets the tests exercise
+// every seam configuration).
+inline int rows_per_segment(const Layout& g, int cx, bool pcg, int forced) {
   if (forced > 0) return forced;
   const int nstrips = (g.Ex + cx - 1) / cx;
   // Plain apply: one element row per workgroup until the grid exceeds 4096
@@ -851,7 +850,7 @@ void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mod
 template <typename T, int P>
 int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
   constexpr int CX = SEM_AX_THREADS / P;
-  const AxPlan pl = make_plan(g, a.assemble != 0, pc != nullptr);
+  const AxPlan pl = make_plan(g, a.assemble != 0, pc != nullptr, a.rows);
   AxP<T> prm = make_params<T, P>(g, a, pl);
   const dim3 grid(pl.nk1), block(CX * P);
   int mode = a.assemble ? 1 : 0;
@@ -869,12 +868,10 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
     mode = 2;
   }
   // 16x16 blocks run their contractions on matrix cores in every precision
-  // (FP64 / FP32: exact MFMA forms); SEM_AX_FMA16=1 selects the FMA path.
+  // (FP64 / FP32: exact MFMA forms); a.fma16 (the SEM_AX_FMA16=1 hook, read
+  // when the operator is created) selects the FMA path.
   int mf = a.mfma;
-  if (mf == 0 && P == 16 && CX * P == 256) {
-    const char* env = getenv("SEM_AX_FMA16");
-    if (!(env && atoi(env) == 1)) mf = sizeof(T) == 8 ? 6 : 5;
-  }
+  if (mf == 0 && P == 16 && CX * P == 256 && !a.fma16) mf = sizeof(T) == 8 ? 6 : 5;
   if (mf == 0) {
     launch_k1<T, P, CX, 0>(st, grid, block, prm, mode);
   } else {

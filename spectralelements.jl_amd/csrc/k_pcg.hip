@@ -423,7 +423,7 @@ struct PcgRun {
     const sem_mesh* m = op->mesh;
     g = m->layout();
     n = (long)m->nl;
-    pl = make_plan(g, true, true);
+    pl = make_plan(g, true, true, op->ax_rows);
     n3 = (int)std::min<long>(SEM_PCG_NPART, std::max<long>(1, (n + 2047) / 2048));
     nQ = pl.nk1;
     sg = SeamGeom{g.P, g.L, g.NC, pl.CX * g.P, pl.nstrips, pl.S * g.P, pl.nsegs, g.per_x,
@@ -651,24 +651,21 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
   const sem_mesh* m = op64->mesh;
   const long n = (long)m->nl;
   hipStream_t st = ctx->stream;
+  // r64, xb (FP64), x32, d32 (FP32), reduction partials: the context's second
+  // workspace (the first holds the inner PCG's vectors)
   double *r64 = nullptr, *xb = nullptr, *nrm = nullptr;
   float *x32 = nullptr, *d32 = nullptr;
-  auto cleanup = [&]() {
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(r64);
-    (void)hipFree(xb);
-    (void)hipFree(x32);
-    (void)hipFree(d32);
-    (void)hipFree(nrm);
-  };
-  if (hipMalloc(&r64, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&xb, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&x32, sizeof(float) * n) != hipSuccess ||
-      hipMalloc(&nrm, sizeof(double) * 260) != hipSuccess ||
-      (d64 && hipMalloc(&d32, sizeof(float) * n) != hipSuccess)) {
-    set_error("sem_pcg_mixed: hipMalloc failed");
-    cleanup();
-    return SEM_ERR_HIP;
+  auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
+  {
+    const size_t b8 = ((sizeof(double) * n + 255) / 256) * 256;
+    const size_t b4 = ((sizeof(float) * n + 255) / 256) * 256;
+    SEM_TRY(ensure_ws2(ctx, 2 * b8 + 2 * b4 + sizeof(double) * 260));
+    char* w0 = static_cast<char*>(ctx->ws2);
+    r64 = reinterpret_cast<double*>(w0);
+    xb = reinterpret_cast<double*>(w0 + b8);
+    x32 = reinterpret_cast<float*>(w0 + 2 * b8);
+    d32 = d64 ? reinterpret_cast<float*>(w0 + 2 * b8 + b4) : nullptr;
+    nrm = reinterpret_cast<double*>(w0 + 2 * b8 + 2 * b4);
   }
   int s = SEM_OK;
   if (d64) s = launch_convert_f64_f32(st, n, d64, d32);

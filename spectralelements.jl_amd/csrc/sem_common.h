@@ -57,6 +57,8 @@ struct sem_ctx {
   hipStream_t stream = nullptr;
   void* ws = nullptr;          // device workspace, grown on demand
   size_t ws_bytes = 0;
+  void* ws2 = nullptr;         // second workspace (mixed-precision solver vectors)
+  size_t ws2_bytes = 0;
   void* pinned = nullptr;      // small pinned host buffer for scalar readback
   size_t pinned_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -109,6 +111,8 @@ struct sem_operator {
   // by 2^e (e chosen per array so the largest magnitude is near 2^14);
   // gscale = 2^-e
   float gscale[4] = {1.f, 1.f, 1.f, 1.f};
+  int ax_rows = 0;    // SEM_AX_ROWS at creation (0: the heuristic)
+  int ax_fma16 = 0;   // SEM_AX_FMA16=1 at creation
   // iterations of the last fresh solve with this operator: the first launch
   // batch of the next one (repeated solves: time steps, adjoints, FD)
   mutable int last_iters = 0;
@@ -117,6 +121,7 @@ struct sem_operator {
 namespace sem {
 
 int ensure_ws(sem_ctx* ctx, size_t bytes);
+int ensure_ws2(sem_ctx* ctx, size_t bytes);
 int ensure_pinned(sem_ctx* ctx, size_t bytes);
 
 // ---- kernel launchers (defined in the .hip translation units)
@@ -146,6 +151,11 @@ struct ApplyArgs {
   int mfma;  // 0: FMA contractions; 1..4: MFMA (FP16, BF16, FP16 x3, BF16 x3)
   // mfma 1, 2: G11, G12, G22, B are FP16 arrays; value = stored * gscale[q]
   float gscale[4];
+  // test hooks, read from the environment when the operator is created
+  // (SEM_AX_ROWS: element rows per workgroup; SEM_AX_FMA16=1: FMA contractions
+  // for 16x16 blocks instead of the exact FP64/FP32 MFMA forms)
+  int rows;
+  int fma16;
 };
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);
@@ -182,7 +192,8 @@ struct AxPlan {
   int CX, nstrips, S, nsegs, nk1, nk2;
   long nxitems, total;
 };
-AxPlan make_plan(const Layout& g, bool assemble, bool pcg);
+// rows > 0 forces the element rows per workgroup
+AxPlan make_plan(const Layout& g, bool assemble, bool pcg, int rows = 0);
 
 struct PcgLaunch {
   const void* r;  // recursive residual

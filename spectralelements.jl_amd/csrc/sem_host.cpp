@@ -38,6 +38,19 @@ int ensure_ws(sem_ctx* ctx, size_t bytes) {
   return SEM_OK;
 }
 
+int ensure_ws2(sem_ctx* ctx, size_t bytes) {
+  if (ctx->ws2_bytes >= bytes) return SEM_OK;
+  if (ctx->ws2) {
+    SEM_HIP(hipStreamSynchronize(ctx->stream));
+    SEM_HIP(hipFree(ctx->ws2));
+    ctx->ws2 = nullptr;
+    ctx->ws2_bytes = 0;
+  }
+  SEM_HIP(hipMalloc(&ctx->ws2, bytes));
+  ctx->ws2_bytes = bytes;
+  return SEM_OK;
+}
+
 int ensure_pinned(sem_ctx* ctx, size_t bytes) {
   if (ctx->pinned_bytes >= bytes) return SEM_OK;
   if (ctx->pinned) {
@@ -139,6 +152,14 @@ int check_square(const sem_mesh* m) {
   return SEM_OK;
 }
 
+// the launch-shape test hooks (environment), read once per operator
+void read_ax_hooks(int* rows, int* fma16) {
+  const char* r = std::getenv("SEM_AX_ROWS");
+  const char* f = std::getenv("SEM_AX_FMA16");
+  *rows = r ? std::max(0, std::atoi(r)) : 0;
+  *fma16 = (f && std::atoi(f) == 1) ? 1 : 0;
+}
+
 int mfma_mode(int precision) {
   switch (precision) {
     case SEM_F16_MFMA: return 1;
@@ -174,6 +195,8 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   a.assemble = op->desc.assemble;
   a.mfma = mfma_mode(op->precision);
   for (int q = 0; q < 4; ++q) a.gscale[q] = op->gscale[q];
+  a.rows = op->ax_rows;
+  a.fma16 = op->ax_fma16;
   return a;
 }
 
@@ -227,6 +250,7 @@ int sem_ctx_destroy(sem_ctx* ctx) {
   if (!ctx) return SEM_OK;
   hipStreamSynchronize(ctx->stream);
   if (ctx->ws) hipFree(ctx->ws);
+  if (ctx->ws2) hipFree(ctx->ws2);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
@@ -527,6 +551,7 @@ int sem_hlmz(sem_ctx* ctx, const sem_mesh* m, const double* u, double nu, const 
   a.has_mass = (k != 0.0 || k_f) ? 1 : 0;
   a.bc = 0;
   a.assemble = 0;
+  read_ax_hooks(&a.rows, &a.fma16);
   return launch_apply<double>(ctx->stream, m->layout(), a);
 }
 
@@ -547,6 +572,7 @@ int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc
   op->precision = precision;
   op->desc = *desc;
   op->has_mass = desc->k != 0.0 || desc->k_f;
+  read_ax_hooks(&op->ax_rows, &op->ax_fma16);
   const double* src[6] = {desc->G11 ? desc->G11 : m->f[SEM_G11],
                           desc->G12 ? desc->G12 : m->f[SEM_G12],
                           desc->G22 ? desc->G22 : m->f[SEM_G22],

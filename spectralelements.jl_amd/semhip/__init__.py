@@ -8,6 +8,7 @@ file:line each function mirrors.
 from ._lib import SemError, load as load_library, LIB_PATH  # noqa: F401
 from .core import (  # noqa: F401
     ABu, Diagonal, GSMap, MG, Mesh, PCGInfo, SEMOperator, adjoint_dnu, advect, ctx, deform_affine01,
+    gordonHall, gordonHallMap, semInterp,
     deform_sinsin, derivMat, empty_field, fixU, gatherScatter, gausslobatto, generateMask, grad,
     gradT, hlmz, identity, interpMat, lapl, mask, mass, ndgrid, norm_inf, pcg, pcg_, pcg_mixed,
     semmesh, synchronize, to_device, to_host, wdot, dot, combine, vardiff_factors, annulus,

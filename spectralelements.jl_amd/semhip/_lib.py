@@ -101,6 +101,9 @@ SIGNATURES = {
     "sem_mg_set_lmax": [_vp, _i, _d],
     "sem_mg_vcycle": [_vp, _vp, _vp, _vp],
     "sem_pcg_mg": [_vp, _vp, _vp, _vp, _vp, _d, _i, C.POINTER(sem_pcg_info)],
+    "sem_gordon_hall": [_vp, _i, _i] + [_vp] * 12,
+    "sem_interp_points": [_vp, _i, _i, _i, _i] + [_vp] * 7 + [_i, _vp, _vp, _d, _i, _vp, _vp],
+    "sem_mesh_interp": [_vp, _vp, _vp, _i, _vp, _vp, _d, _i, _vp, _vp],
     "sem_wdot": [_vp, _vp, _vp, _vp, _dp],
     "sem_norm_inf": [_vp, _i64, _vp, _dp],
     "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],

@@ -910,3 +910,81 @@ def pcg_mixed(b, op64: SEMOperator, op_lo: SEMOperator, d=None, tol: float = 1e-
     if st == L.SEM_WARN_MAXITER:
         L.maxiter_warning(pi.res)
     return x
+
+
+# ------------------------------------------ mesh generation and evaluation
+def _dev_vec(v) -> torch.Tensor:
+    if isinstance(v, torch.Tensor):
+        return v.detach().to(device="cuda", dtype=_F64).contiguous().reshape(-1)
+    return torch.as_tensor(np.asarray(v, dtype=np.float64).reshape(-1)).cuda()
+
+
+def gordonHall(xrm, xrp, xsm, xsp, yrm, yrp, ysm, ysp, zr, zs):
+    """Transfinite interpolation of four boundary curves (geom.jl:8-31) on
+    the GPU: xrm, xrp, yrm, yrp sample the r = -1 and r = +1 edges at the
+    points zs; xsm, xsp, ysm, ysp the s = -1 and s = +1 edges at zr. Returns
+    the (len(zr), len(zs)) fields x, y (column-major device arrays)."""
+    zr, zs = _dev_vec(zr), _dev_vec(zs)
+    e = [_dev_vec(v) for v in (xrm, xrp, xsm, xsp, yrm, yrp, ysm, ysp)]
+    nr, ns = zr.numel(), zs.numel()
+    for v, n in zip(e, (ns, ns, nr, nr, ns, ns, nr, nr)):
+        if v.numel() != n:
+            raise ValueError("gordonHall: r-edges need len(zs) values, s-edges len(zr)")
+    x, y = empty_field(nr, ns), empty_field(nr, ns)
+    L.call("sem_gordon_hall", ctx(), nr, ns, _p(zr), _p(zs), *[_p(v) for v in e], _p(x), _p(y))
+    return x, y
+
+
+class gordonHallMap:
+    """A mesh deformation (the deform argument of Mesh) by transfinite
+    interpolation of four boundary curves of the domain. Each curve is a
+    callable t -> (x(t), y(t)) on t in [-1, 1]: rm / rp the r = -1 / +1
+    sides (t = s), sm / sp the s = -1 / +1 sides (t = r). The undeformed
+    node coordinates (semmesh, in [-1, 1]) are the (r, s) of the map."""
+
+    def __init__(self, rm, rp, sm, sp):
+        self.rm, self.rp, self.sm, self.sp = rm, rp, sm, sp
+
+    def __call__(self, x, y):
+        r = x[:, 0].contiguous()
+        s = y[0, :].contiguous()
+        xrm, yrm = self.rm(s)
+        xrp, yrp = self.rp(s)
+        xsm, ysm = self.sm(r)
+        xsp, ysp = self.sp(r)
+        return gordonHall(xrm, xrp, xsm, xsp, yrm, yrp, ysm, ysp, r, s)
+
+
+def semInterp(u, *args, tol: float = 1e-8, maxiter: int = 50):
+    """Field value at physical points (interp.jl:47-102).
+
+    semInterp(u, xp, yp, msh): u a nodal field of msh, xp / yp arrays of
+    points; each point is located in the element whose map inverts onto it
+    (Newton from the closest GLL node); returns a device array of values,
+    NaN for points in no element.
+    semInterp(u, xp, yp, rxe, rye, sxe, sye, zr, zs, xe, ye[, tol, maxiter]):
+    the reference's one-element, one-point form; returns a float."""
+    if len(args) == 3 and isinstance(args[2], Mesh):
+        xp, yp, msh = args
+        xpd, ypd = _dev_vec(xp), _dev_vec(yp)
+        if xpd.numel() != ypd.numel():
+            raise ValueError("semInterp: xp and yp differ in length")
+        ud = to_device(u)
+        out = torch.empty(xpd.numel(), dtype=_F64, device="cuda")
+        L.call("sem_mesh_interp", ctx(), msh.h, _p(ud), xpd.numel(), _p(xpd), _p(ypd),
+               float(tol), int(maxiter), _p(out), None)
+        return out
+    if len(args) in (10, 12):
+        xp, yp, rxe, rye, sxe, sye, zr, zs, xe, ye = args[:10]
+        if len(args) == 12:
+            tol, maxiter = args[10], args[11]
+        f = [to_device(np.asarray(v.cpu() if isinstance(v, torch.Tensor) else v, dtype=np.float64))
+             for v in (u, xe, ye, rxe, rye, sxe, sye)]
+        nr, ns = f[0].shape
+        pts = torch.tensor([[float(xp)], [float(yp)]], dtype=_F64, device="cuda")
+        out = torch.empty(1, dtype=_F64, device="cuda")
+        L.call("sem_interp_points", ctx(), nr, ns, 1, 1, _p(f[0]), _p(f[1]), _p(f[2]), _p(f[3]),
+               _p(f[4]), _p(f[5]), _p(f[6]), 1, _p(pts[0]), _p(pts[1]), float(tol),
+               int(maxiter), _p(out), None)
+        return float(out.item())
+    raise TypeError("semInterp(u, xp, yp, msh) or the reference's 12-argument form")
