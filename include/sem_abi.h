@@ -158,6 +158,13 @@ int sem_mass(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);
 int sem_abu(sem_ctx* ctx, int m, int n, const double* u, const double* As, int ma,
             int na, const double* Br, int mb, int nb, int trans, double* y);
 
+/* elementwise y = a .* b, a ./ b or alpha ./ a over n entries (y may alias a
+ * or b): the products, quotients and inverse of the diagonal operator of the
+ * Spectral submodule (DiagonalOp, Spectral/Operators.jl:37-118) */
+enum sem_pointwise_op { SEM_PW_MUL = 0, SEM_PW_DIV = 1, SEM_PW_SCALAR_DIV = 2 };
+int sem_pointwise(sem_ctx* ctx, int op, int64_t n, double alpha, const double* a,
+                  const double* b, double* y);
+
 /* ------------------------------------------------------ element operators */
 /* lapl(u, msh), lapl.jl:26-36 (element-local weak Laplacian, no QQ') */
 int sem_lapl(sem_ctx* ctx, const sem_mesh* mesh, const double* u, double* y);

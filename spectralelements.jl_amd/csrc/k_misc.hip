@@ -326,6 +326,16 @@ __global__ void k_gprod(const double* __restrict__ G11, const double* __restrict
   ws[t] = G12[t] * av + G22[t] * bv;
 }
 
+// elementwise y = a .* b (op 0), a ./ b (op 1), alpha ./ a (op 2); y may
+// alias a or b
+__global__ void k_pointwise(int op, double alpha, const double* a, const double* b, double* y,
+                            long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double av = a[t];
+  y[t] = op == 0 ? av * b[t] : (op == 1 ? av / b[t] : alpha / av);
+}
+
 // out = alpha x + beta y (out may alias x or y)
 __global__ void k_axpby(double alpha, const double* x, double beta, const double* y, double* out,
                         long n) {
@@ -734,6 +744,13 @@ int launch_gprod(hipStream_t st, int64_t n, const double* G11, const double* G12
 int launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta,
                  const double* y, double* out) {
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, st, alpha, x, beta, y, out, (long)n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_pointwise(hipStream_t st, int op, int64_t n, double alpha, const double* a,
+                     const double* b, double* y) {
+  hipLaunchKernelGGL(k_pointwise, dim3(nblk(n)), dim3(256), 0, st, op, alpha, a, b, y, (long)n);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

@@ -221,6 +221,8 @@ int launch_vardiff(hipStream_t st, const sem_mesh* m, const double* nu, double* 
                    double* G22);
 int launch_gprod(hipStream_t st, int64_t n, const double* G11, const double* G12,
                  const double* G22, const double* a, const double* b, double* wr, double* ws);
+int launch_pointwise(hipStream_t st, int op, int64_t n, double alpha, const double* a,
+                     const double* b, double* y);
 int launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta,
                  const double* y, double* out);
 

@@ -744,6 +744,14 @@ int sem_axpby(sem_ctx* ctx, int64_t n, double alpha, const double* x, double bet
   return launch_axpby(ctx->stream, n, alpha, x, beta, y, out);
 }
 
+int sem_pointwise(sem_ctx* ctx, int op, int64_t n, double alpha, const double* a,
+                  const double* b, double* y) {
+  SEM_ARG(ctx && a && y && n >= 0, "NULL argument");
+  SEM_ARG(op >= SEM_PW_MUL && op <= SEM_PW_SCALAR_DIV, "sem_pointwise: unknown op %d", op);
+  SEM_ARG(op == SEM_PW_SCALAR_DIV || b, "sem_pointwise: op %d needs b", op);
+  return launch_pointwise(ctx->stream, op, n, alpha, a, b, y);
+}
+
 int sem_vardiff_factors(sem_ctx* ctx, const sem_mesh* m, const double* nu, double* G11,
                         double* G12, double* G22) {
   SEM_ARG(ctx && m && nu && G11 && G12 && G22, "NULL argument");

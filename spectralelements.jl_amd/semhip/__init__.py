@@ -21,3 +21,4 @@ from .cd import (  # noqa: F401
 )
 from .diffusion import Diffusion  # noqa: F401
 from . import workloads  # noqa: F401
+from . import sciml  # noqa: F401
