@@ -500,6 +500,10 @@ int build(sem_ctx* ctx, sem_mg* mg) {
 
 int check_op(const sem_operator* op) {
   SEM_ARG(op->precision == SEM_F64, "multigrid: the operator must be SEM_F64");
+  if (op->generic) {
+    set_error("multigrid: needs nr == ns <= %d", kMaxP);
+    return SEM_ERR_UNSUPPORTED;
+  }
   SEM_ARG(op->desc.assemble, "multigrid: the operator must assemble (desc.assemble = 1)");
   if (op->desc.nu_f || op->desc.k_f || op->desc.G11 || op->desc.G12 || op->desc.G22 ||
       op->desc.B) {

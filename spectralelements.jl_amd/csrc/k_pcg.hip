@@ -336,22 +336,24 @@ __global__ void k_jacobi_local(const double* __restrict__ G11, const double* __r
                                const double* __restrict__ G22, const double* __restrict__ B,
                                const double* __restrict__ Dr, const double* __restrict__ Ds,
                                double nu, const double* __restrict__ nu_f, double kc,
-                               const double* __restrict__ k_f, int has_mass, int P, int L, long n,
-                               double* __restrict__ dl) {
+                               const double* __restrict__ k_f, int has_mass, int nr, int ns,
+                               int L, long n, double* __restrict__ dl) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = (int)(t % L);
   const int j = (int)(t / L);
-  const int a = i % P, i0 = i - a;
-  const int b = j % P, j0 = j - b;
+  const int a = i % nr, i0 = i - a;
+  const int b = j % ns, j0 = j - b;
   double t1 = 0, t3 = 0;
-  for (int q = 0; q < P; ++q) {
-    const double dr = Dr[q + P * a];
+  for (int q = 0; q < nr; ++q) {
+    const double dr = Dr[q + nr * a];
     t1 += dr * dr * G11[(i0 + q) + (long)L * j];
-    const double ds = Ds[q + P * b];
+  }
+  for (int q = 0; q < ns; ++q) {
+    const double ds = Ds[q + ns * b];
     t3 += ds * ds * G22[i + (long)L * (j0 + q)];
   }
-  const double t2 = 2.0 * Dr[a + P * a] * Ds[b + P * b] * G12[t];
+  const double t2 = 2.0 * Dr[a + nr * a] * Ds[b + ns * b] * G12[t];
   const double nuv = nu_f ? nu_f[t] : nu;
   double v = nuv * (t1 + t2 + t3);
   if (has_mass) v = v + (k_f ? k_f[t] : kc) * B[t];
@@ -889,7 +891,7 @@ int launch_jacobi_local(hipStream_t st, const sem_operator* op, double* dl) {
   const double* B = ds.B ? ds.B : m->f[SEM_B];
   hipLaunchKernelGGL(k_jacobi_local, dim3(nblk(m->nl)), dim3(256), 0, st, G11, G12, G22, B,
                      m->Dr, m->Ds, ds.nu, ds.nu_f, ds.k, ds.k_f, op->has_mass ? 1 : 0, m->nr,
-                     m->L, (long)m->nl, dl);
+                     m->ns, m->L, (long)m->nl, dl);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }
@@ -904,6 +906,9 @@ int launch_inv_masked(hipStream_t st, const Layout& g, int bits, const double* d
 int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,
             void* x, double tol, int maxiter, sem_pcg_info* info) {
   if (precond == SEM_PRECOND_IDENTITY) d = nullptr;
+  if (op->generic)
+    return pcg_generic(ctx, op, static_cast<const double*>(d), static_cast<const double*>(b),
+                       static_cast<double*>(x), tol, maxiter, info);
   if (op->precision == SEM_F64)
     return pcg_run_t<double>(ctx, op, static_cast<const double*>(d),
                              static_cast<const double*>(b), static_cast<double*>(x), tol, maxiter,
@@ -926,6 +931,10 @@ extern "C" int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond
   SEM_ARG(maxiter >= 0, "sem_pcg_noalloc: maxiter < 0");
   SEM_ARG(tol >= 0.0, "sem_pcg_noalloc: tol < 0");
   if (precond == SEM_PRECOND_IDENTITY) d = nullptr;
+  if (op->generic) {
+    set_error("sem_pcg_noalloc: meshes with nr != ns or nr > %d take sem_pcg", kMaxP);
+    return SEM_ERR_UNSUPPORTED;
+  }
   if (op->precision == SEM_F64)
     return pcg_noalloc_t<double>(ctx, op, static_cast<const double*>(d),
                                  static_cast<const double*>(b), static_cast<double*>(x), tol,
@@ -943,6 +952,10 @@ extern "C" int sem_pcg_mixed(sem_ctx* ctx, const sem_operator* op64, const sem_o
   SEM_ARG(op_lo->precision != SEM_F64, "sem_pcg_mixed: op_lo must be a low-precision operator");
   SEM_ARG(op64->mesh == op_lo->mesh, "sem_pcg_mixed: operators on different meshes");
   SEM_ARG(op64->desc.assemble && op_lo->desc.assemble, "sem_pcg_mixed: operators must assemble");
+  if (op64->generic || op_lo->generic) {
+    set_error("sem_pcg_mixed: needs nr == ns <= %d", kMaxP);
+    return SEM_ERR_UNSUPPORTED;
+  }
   SEM_ARG(max_outer >= 1 && max_inner >= 1 && inner_tol > 0 && inner_tol < 1 && tol >= 0,
           "sem_pcg_mixed: bad limits");
   return mixed_run(ctx, op64, op_lo, d64, b, x, tol, inner_tol, max_outer, max_inner, info);

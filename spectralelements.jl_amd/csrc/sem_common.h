@@ -116,6 +116,9 @@ struct sem_operator {
   // iterations of the last fresh solve with this operator: the first launch
   // batch of the next one (repeated solves: time steps, adjoints, FD)
   mutable int last_iters = 0;
+  // nr != ns or nr > 16: the element-per-workgroup kernel and the map-based
+  // QQ' instead of the fused strip kernels (FP64 only)
+  bool generic = false;
 };
 
 namespace sem {
@@ -260,6 +263,16 @@ int launch_advect(hipStream_t st, const sem_mesh* mV, const sem_mesh* mD, const 
 
 // Looks up this GPU's scratch array of the allocation-free PCG (device current)
 int init_scratch(int device);
+
+// generic path (k_generic.hip): any nr, ns. Element-local
+// nu A u + k B u (+ QQ' and the mask when assemble) with op's factors or the
+// mesh's (nu_f, k_f fields optional), and PCG with it
+int launch_apply_generic(hipStream_t st, const sem_mesh* m, const double* u, double* y,
+                         const double* G11, const double* G12, const double* G22,
+                         const double* B, double nu, const double* nu_f, double k,
+                         const double* k_f, int has_mass, int assemble, int bits);
+int pcg_generic(sem_ctx* ctx, const sem_operator* op, const double* d, const double* b, double* x,
+                double tol, int maxiter, sem_pcg_info* info);
 
 // PCG driver (k_pcg.hip)
 int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,

@@ -136,20 +136,10 @@ bool gll_table_matches(int P, const double* D) {
   }
 }
 
-int check_square(const sem_mesh* m) {
-  if (m->nr != m->ns) {
-    set_error("fused operators need nr == ns (got %d, %d)", m->nr, m->ns);
-    return SEM_ERR_UNSUPPORTED;
-  }
-  if (m->nr < 2 || m->nr > kMaxP) {
-    set_error("fused operators support 2 <= nr <= %d (got %d)", kMaxP, m->nr);
-    return SEM_ERR_UNSUPPORTED;
-  }
-  if (!gll_table_matches(m->nr, m->hDr.data())) {
-    set_error("internal: compile-time GLL table differs from Mesh.Dr for nr = %d", m->nr);
-    return SEM_ERR_UNSUPPORTED;
-  }
-  return SEM_OK;
+// The fused strip kernels take nr == ns <= 16 (compile-time D tables, equal
+// to Mesh.Dr); other meshes run the generic element kernel.
+bool fused_mesh(const sem_mesh* m) {
+  return m->nr == m->ns && m->nr >= 2 && m->nr <= kMaxP && gll_table_matches(m->nr, m->hDr.data());
 }
 
 // the launch-shape test hooks (environment), read once per operator
@@ -534,7 +524,10 @@ int sem_lapl(sem_ctx* ctx, const sem_mesh* m, const double* u, double* y) {
 int sem_hlmz(sem_ctx* ctx, const sem_mesh* m, const double* u, double nu, const double* nu_f,
              double k, const double* k_f, double* y) {
   SEM_ARG(ctx && m && u && y, "NULL argument");
-  SEM_TRY(check_square(m));
+  if (!fused_mesh(m))
+    return launch_apply_generic(ctx->stream, m, u, y, m->f[SEM_G11], m->f[SEM_G12],
+                                m->f[SEM_G22], m->f[SEM_B], nu, nu_f, k, k_f,
+                                (k != 0.0 || k_f) ? 1 : 0, 0, 0);
   ApplyArgs a{};
   a.u = u;
   a.y = y;
@@ -558,8 +551,13 @@ int sem_hlmz(sem_ctx* ctx, const sem_mesh* m, const double* u, double nu, const 
 int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc, int precision,
                         sem_operator** out) {
   SEM_ARG(ctx && m && desc && out, "NULL argument");
-  SEM_TRY(check_square(m));
   SEM_ARG(precision >= SEM_F64 && precision <= SEM_BF16X3_MFMA, "bad precision %d", precision);
+  const bool generic = !fused_mesh(m);
+  if (generic && precision != SEM_F64) {
+    set_error("meshes with nr != ns or nr > %d have FP64 operators only (got nr %d, ns %d)",
+              kMaxP, m->nr, m->ns);
+    return SEM_ERR_UNSUPPORTED;
+  }
   if (precision >= SEM_F16_MFMA) {
     if (m->nr != 16) {
       set_error("MFMA operators need nr == ns == 16 (got %d)", m->nr);
@@ -571,6 +569,7 @@ int sem_operator_create(sem_ctx* ctx, const sem_mesh* m, const sem_op_desc* desc
   op->mesh = m;
   op->precision = precision;
   op->desc = *desc;
+  op->generic = generic;
   op->has_mass = desc->k != 0.0 || desc->k_f;
   read_ax_hooks(&op->ax_rows, &op->ax_fma16);
   const double* src[6] = {desc->G11 ? desc->G11 : m->f[SEM_G11],
@@ -661,6 +660,13 @@ int sem_operator_precision(const sem_operator* op, int* precision) {
 int sem_operator_apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y) {
   SEM_ARG(ctx && op && u && y, "NULL argument");
   SEM_ARG(u != y, "sem_operator_apply: u and y must not alias");
+  if (op->generic)
+    return launch_apply_generic(
+        ctx->stream, op->mesh, static_cast<const double*>(u), static_cast<double*>(y),
+        static_cast<const double*>(op->G11), static_cast<const double*>(op->G12),
+        static_cast<const double*>(op->G22), static_cast<const double*>(op->B), op->desc.nu,
+        static_cast<const double*>(op->nu_f), op->desc.k, static_cast<const double*>(op->k_f),
+        op->has_mass ? 1 : 0, op->desc.assemble, op->desc.assemble ? op->desc.bc_bits : 0);
   const ApplyArgs a = args_of(op, u, y);
   if (op->precision == SEM_F64) return launch_apply<double>(ctx->stream, op->mesh->layout(), a);
   return launch_apply<float>(ctx->stream, op->mesh->layout(), a);
@@ -694,7 +700,10 @@ int sem_wdot(sem_ctx* ctx, const sem_mesh* m, const double* a, const double* b, 
   const int np = 256;
   SEM_TRY(ensure_ws(ctx, sizeof(double) * (np + 1)));
   double* part = static_cast<double*>(ctx->ws);
-  SEM_TRY(launch_wdot(ctx->stream, m->layout(), a, b, part, np, part + np));
+  if (m->nr == m->ns)
+    SEM_TRY(launch_wdot(ctx->stream, m->layout(), a, b, part, np, part + np));
+  else
+    SEM_TRY(launch_dot(ctx->stream, m->nl, a, b, m->f[SEM_MULT], part, np, part + np));
   SEM_HIP(hipMemcpyAsync(result, part + np, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   SEM_HIP(hipStreamSynchronize(ctx->stream));
   return SEM_OK;
