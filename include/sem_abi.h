@@ -32,6 +32,8 @@ extern "C" {
 enum sem_status {
   SEM_OK = 0,
   SEM_WARN_MAXITER = 1,
+  SEM_WARN_STALLED = 2, /* sem_pcg_mixed: stopped after four replacements
+                           without a 10% gain (see there) */
   SEM_ERR_ARG = -1,
   SEM_ERR_HIP = -2,
   SEM_ERR_UNSUPPORTED = -3,
@@ -72,9 +74,16 @@ enum sem_bc_bits { SEM_BC_XMIN = 1, SEM_BC_XMAX = 2, SEM_BC_YMIN = 4, SEM_BC_YMA
 
 enum sem_precond {
   SEM_PRECOND_IDENTITY = 0, /* opPrecond of Diffusion, diffusion.jl:47-49 */
-  SEM_PRECOND_DIAG = 1      /* z = d .* r with a caller/operator diagonal d: the
-                               Jacobi inverse (sem_operator_jacobi) or
-                               1 ./ (B * b0) of convectionDiffusion.jl:87-91 */
+  SEM_PRECOND_DIAG = 1,     /* z = d .* r with a diagonal d that is equal on all
+                               copies of a shared node: the Jacobi inverse
+                               (sem_operator_jacobi) or any assembled field */
+  SEM_PRECOND_DIAG_LOCAL = 2 /* z = d .* r with a d that may differ between the
+                               copies of a node, as 1 ./ (B * b0) with the
+                               element-local B of convectionDiffusion.jl:87-91:
+                               z is then discontinuous, so <p, A p> is formed on
+                               the assembled A p as pcg.jl:50 does (the fused
+                               solver's element-local pairing needs a continuous
+                               p). sem_pcg only, SEM_F64 operators. */
 };
 
 typedef struct sem_ctx sem_ctx;
@@ -226,8 +235,9 @@ typedef struct sem_pcg_info {
  * x0 = 0 (pcg.jl:25; pcg! ignores the incoming x, :71), stop when
  * norm(r, Inf) <= tol (absolute, :36), mult-weighted inner products with the
  * operator mesh's mult (:45, :52), SEM_WARN_MAXITER at k == maxiter (:39).
- * precond: SEM_PRECOND_IDENTITY or SEM_PRECOND_DIAG with the device diagonal
- * d (z = d .* r). For SEM_F32 operators b, x and d are float*. */
+ * precond: SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG or SEM_PRECOND_DIAG_LOCAL
+ * with the device diagonal d (z = d .* r). For SEM_F32 operators b, x and d
+ * are float*. x must not alias b. */
 int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
             const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
 
@@ -259,8 +269,11 @@ int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond, const voi
  * restart it at every replacement; the single-pass BF16 operator always restarts
  * it (classical defect correction).
  * SEM_MIXED_RELIABLE=1 / 0 forces one policy. Stops when norm(b - op64(x), Inf) <= tol;
- * SEM_WARN_MAXITER after max_outer replacements, four replacements without a
- * 10% improvement, or max_inner total inner iterations. d64: FP64 Jacobi
+ * SEM_WARN_MAXITER after max_outer replacements or max_inner total inner
+ * iterations; SEM_WARN_STALLED after four replacements without a 10%
+ * improvement (the low-precision operator's error times the condition number
+ * is then too large for defect correction to contract). Either way x holds
+ * the best iterate and info->res its true residual. d64: FP64 Jacobi
  * diagonal (or NULL for identity). info->outer = replacements, info->iters =
  * inner iterations. Environment SEM_DEBUG_MIXED=1 prints one line per cycle
  * to stderr. */

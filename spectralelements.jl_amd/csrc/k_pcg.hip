@@ -105,9 +105,10 @@ __global__ void k_pcg_tables(int* __restrict__ rinfo, int2* __restrict__ cinfo, 
   if (t < g.NC) cinfo[t] = col_info(t, g);
 }
 
-// rows whose fact table is copied into LDS (64 KiB); longer rows read it
-// from global memory
-constexpr int kRowTable = 16384;
+// rows whose fact table is copied into LDS (56 KiB, so that with the 8 KiB
+// reduction array the block stays within 64 KiB); longer rows read it from
+// global memory
+constexpr int kRowTable = 14336;
 
 // One node of the PCG update: r -= alpha wf, partials of <r, z>_mult (m =
 // the node's mult) and norm(r, Inf). With keep == false the node is left as
@@ -776,7 +777,11 @@ int mixed_run(sem_ctx* ctx, const sem_operator* op64, const sem_operator* op_lo,
       // a replacement without progress: the kept direction has gone stale,
       // restart the Krylov space from here on
       if (stall > 0 && reliable && !menv) reliable = false;
-      if (cycles >= max_cycles || stall >= 4 || total >= max_inner) break;
+      if (cycles >= max_cycles || total >= max_inner) break;
+      if (stall >= 4) {
+        status = SEM_WARN_STALLED;
+        break;
+      }
       if (bad || !reliable) {
         s = start_inner();
       } else {
@@ -906,7 +911,7 @@ int launch_inv_masked(hipStream_t st, const Layout& g, int bits, const double* d
 int pcg_run(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, const void* b,
             void* x, double tol, int maxiter, sem_pcg_info* info) {
   if (precond == SEM_PRECOND_IDENTITY) d = nullptr;
-  if (op->generic)
+  if (op->generic || precond == SEM_PRECOND_DIAG_LOCAL)
     return pcg_generic(ctx, op, static_cast<const double*>(d), static_cast<const double*>(b),
                        static_cast<double*>(x), tol, maxiter, info);
   if (op->precision == SEM_F64)
@@ -928,6 +933,7 @@ extern "C" int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond
   SEM_ARG(op->desc.assemble, "sem_pcg_noalloc needs an assembled operator (desc.assemble = 1)");
   SEM_ARG(precond == SEM_PRECOND_IDENTITY || (precond == SEM_PRECOND_DIAG && d),
           "sem_pcg_noalloc: SEM_PRECOND_DIAG needs the diagonal d");
+  SEM_ARG(x != b, "sem_pcg_noalloc: x must not alias b");
   SEM_ARG(maxiter >= 0, "sem_pcg_noalloc: maxiter < 0");
   SEM_ARG(tol >= 0.0, "sem_pcg_noalloc: tol < 0");
   if (precond == SEM_PRECOND_IDENTITY) d = nullptr;

@@ -688,8 +688,12 @@ int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, co
             void* x, double tol, int maxiter, sem_pcg_info* info) {
   SEM_ARG(ctx && op && b && x, "NULL argument");
   SEM_ARG(op->desc.assemble, "sem_pcg needs an assembled operator (desc.assemble = 1)");
-  SEM_ARG(precond == SEM_PRECOND_IDENTITY || (precond == SEM_PRECOND_DIAG && d),
-          "sem_pcg: SEM_PRECOND_DIAG needs the diagonal d");
+  SEM_ARG(precond == SEM_PRECOND_IDENTITY ||
+              ((precond == SEM_PRECOND_DIAG || precond == SEM_PRECOND_DIAG_LOCAL) && d),
+          "sem_pcg: SEM_PRECOND_DIAG(_LOCAL) needs the diagonal d");
+  SEM_ARG(x != b, "sem_pcg: x must not alias b");
+  SEM_ARG(precond != SEM_PRECOND_DIAG_LOCAL || op->precision == SEM_F64,
+          "sem_pcg: SEM_PRECOND_DIAG_LOCAL needs an SEM_F64 operator");
   SEM_ARG(maxiter >= 0, "sem_pcg: maxiter < 0");
   SEM_ARG(tol >= 0.0, "sem_pcg: tol < 0");
   return pcg_run(ctx, op, precond, d, b, x, tol, maxiter, info);

@@ -22,12 +22,14 @@ const libsemhip = get(ENV, "SEMHIP_LIB", joinpath(@__DIR__, "..", "lib", "libsem
 
 const SEM_OK = Cint(0)
 const SEM_WARN_MAXITER = Cint(1)
+const SEM_WARN_STALLED = Cint(2)
 const SEM_F64 = Cint(0)
 const SEM_F32 = Cint(1)
 const SEM_F16_MFMA = Cint(2)
 const SEM_BF16_MFMA = Cint(3)
 const SEM_PRECOND_IDENTITY = Cint(0)
 const SEM_PRECOND_DIAG = Cint(1)
+const SEM_PRECOND_DIAG_LOCAL = Cint(2)
 const SEM_DEFORM_NONE = Cint(0)
 const SEM_DEFORM_SINSIN = Cint(1)
 const SEM_B = Cint(8)
@@ -38,6 +40,7 @@ lasterror() = unsafe_string(ccall((:sem_last_error, libsemhip), Cstring, ()))
 function check(st::Cint)
     st == SEM_OK && return st
     st == SEM_WARN_MAXITER && return st   # pcg.jl:39 warns and returns
+    st == SEM_WARN_STALLED && return st   # mixed solve: no progress, best iterate kept
     error("libsemhip: ", lasterror())
 end
 

@@ -13,12 +13,12 @@ import warnings
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsemhip.so")
 
-SEM_OK, SEM_WARN_MAXITER = 0, 1
+SEM_OK, SEM_WARN_MAXITER, SEM_WARN_STALLED = 0, 1, 2
 SEM_ERR = {-1: "SEM_ERR_ARG", -2: "SEM_ERR_HIP", -3: "SEM_ERR_UNSUPPORTED",
            -4: "SEM_ERR_NONFINITE", -5: "SEM_ERR_BREAKDOWN"}
 SEM_F64, SEM_F32, SEM_F16_MFMA, SEM_BF16_MFMA = 0, 1, 2, 3
 SEM_DEFORM_NONE, SEM_DEFORM_SINSIN, SEM_DEFORM_AFFINE01 = 0, 1, 2
-SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG = 0, 1
+SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG, SEM_PRECOND_DIAG_LOCAL = 0, 1, 2
 SEM_PW_MUL, SEM_PW_DIV, SEM_PW_SCALAR_DIV = 0, 1, 2
 FIELD_IDS = {n: i for i, n in enumerate(
     ["x", "y", "Jac", "Jaci", "rx", "ry", "sx", "sy", "B", "Bi", "G11", "G12", "G22", "mult"])}
@@ -157,6 +157,15 @@ def check(status: int, what: str = "") -> int:
 
 def call(name: str, *args) -> int:
     return check(getattr(load(), name)(*args), name)
+
+
+class StallWarning(RuntimeWarning):
+    """The mixed-precision solver stopped making progress before tol."""
+
+
+def stall_warning(res: float):
+    warnings.warn(f"pcg_mixed stalled: res:{res} (the low-precision operator is too "
+                  "inaccurate for this problem's conditioning)", StallWarning, stacklevel=3)
 
 
 def maxiter_warning(res: float):

@@ -20,8 +20,10 @@ Differences from the reference, all value-preserving or opt-in:
 * The default preconditioner is the assembled Jacobi diagonal of the step
   operator. ``precond="reference"`` selects u ./ B ./ bdfB[1]
   (convectionDiffusion.jl:87-91), whose per-copy B makes PCG iterates
-  discontinuous at shared nodes; the adjoint is exact only for the continuous
-  preconditioners.
+  discontinuous at shared nodes: that solve forms <p, A p> on the assembled
+  A p as pcg.jl:50 does (SEM_PRECOND_DIAG_LOCAL), so it runs the reference's
+  recurrence, with host-read scalars each iteration; the adjoint is exact only
+  for the continuous preconditioners.
 * The PCG stopping test is relative, norm(r, Inf) <= tol * norm(rhs, Inf),
   unless ``abs_tol`` is given (the reference's pcg! default is tol = 1e-8
   absolute, pcg.jl:16).
@@ -157,8 +159,11 @@ class ConvectionDiffusion:
         e = self._ops.get(b0)
         if e is None:
             op = SEMOperator(self.nu, b0, self.mshV, self.fld.M)
-            d = op.jacobi() if self.precond == "jacobi" else 1.0 / (self.mshV.B * b0)
-            e = self._ops[b0] = (op, Diagonal(d))
+            if self.precond == "jacobi":
+                e = (op, Diagonal(op.jacobi(), local=False))
+            else:
+                e = (op, Diagonal(1.0 / (self.mshV.B * b0), local=True))
+            self._ops[b0] = e
         return e
 
     def opLHS(self, u):

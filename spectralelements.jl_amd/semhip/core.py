@@ -702,10 +702,27 @@ class SEMOperator:
 
 
 class Diagonal:
-    """Diagonal preconditioner z = d .* r (callable like the reference's opM)."""
+    """Diagonal preconditioner z = d .* r (callable like the reference's opM).
+    local: True when d may differ between the copies of a shared node (as
+    1 ./ (B * b0) with the element-local B, convectionDiffusion.jl:87-91), so
+    that z is not continuous; None: decided once per mesh by checking
+    QQ'(mult .* d) == d."""
 
-    def __init__(self, d):
+    def __init__(self, d, local: Optional[bool] = None):
         self.d = to_device(d)
+        self.local = local
+        self._local_on = {}
+
+    def is_local(self, msh) -> bool:
+        if self.local is not None:
+            return self.local
+        key = id(msh)
+        if key not in self._local_on:
+            d = self.d.to(_F64)
+            a = gatherScatter(d * msh.mult, msh)
+            scale = float(d.abs().max()) or 1.0
+            self._local_on[key] = bool(float((a - d).abs().max()) > 1e-12 * scale)
+        return self._local_on[key]
 
     def __call__(self, r):
         return self.d * r
@@ -764,6 +781,7 @@ class PCGInfo:
         self.converged = False
         self.elapsed_ms = 0.0
         self.outer = 0
+        self.stalled = False
 
 
 def _is_identity(opM) -> bool:
@@ -812,7 +830,8 @@ def pcg(b, opA, opM=None, mult=None, ifv: bool = False, tol: float = 1e-8,
         pi = L.sem_pcg_info()
         if isinstance(opM, Diagonal):
             d = opM.d if opA.dtype == _F64 else opM.d.to(torch.float32)
-            st = L.call("sem_pcg", ctx(), opA.h, L.SEM_PRECOND_DIAG, _p(d), _p(bd), _p(xd),
+            kind = L.SEM_PRECOND_DIAG_LOCAL if opM.is_local(opA.msh) else L.SEM_PRECOND_DIAG
+            st = L.call("sem_pcg", ctx(), opA.h, kind, _p(d), _p(bd), _p(xd),
                         float(tol), int(min(maxiter, 2**31 - 1)), C.byref(pi))
         else:
             st = L.call("sem_pcg", ctx(), opA.h, L.SEM_PRECOND_IDENTITY, None, _p(bd), _p(xd),
@@ -907,8 +926,11 @@ def pcg_mixed(b, op64: SEMOperator, op_lo: SEMOperator, d=None, tol: float = 1e-
     if info is not None:
         info.iters, info.res, info.converged = pi.iters, pi.res, bool(pi.converged)
         info.elapsed_ms, info.outer = pi.elapsed_ms, pi.outer
+        info.stalled = st == L.SEM_WARN_STALLED
     if st == L.SEM_WARN_MAXITER:
         L.maxiter_warning(pi.res)
+    elif st == L.SEM_WARN_STALLED:
+        L.stall_warning(pi.res)
     return x
 
 

@@ -487,6 +487,52 @@ def test_mixed_solve_reaches_fp64_tolerance(se, prec):
     assert _rel(x, x64) < 1e-8
 
 
+def test_mixed_solve_reports_stall_distinctly(se):
+    """A tolerance below the FP64 rounding floor cannot be met: the solver
+    stops after four replacements without progress with SEM_WARN_STALLED
+    (a StallWarning, info.stalled), not the maxiter warning, and returns its
+    best iterate."""
+    from semhip import _lib as SL
+    _log("mixed stall")
+    m, mo = _pair(se, 16, 6, (False, False), "sinsin")
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(5)
+    b = mo.gs(Mo * o.mass(rng.standard_normal(mo.x.shape), mo))
+    op64 = se.SEMOperator(1.0, 1.0, m, M)
+    op32 = se.SEMOperator(1.0, 1.0, m, M, precision="f32")
+    info = se.PCGInfo()
+    with pytest.warns(SL.StallWarning):
+        x = se.to_host(se.pcg_mixed(se.to_device(b), op64, op32, d=op64.jacobi(), tol=0.0,
+                                    inner_tol=1e-5, max_outer=200, info=info))
+    assert info.stalled and not info.converged and info.outer < 200
+    assert np.abs(b - o.opLHS(x, 1.0, 1.0, mo, Mo)).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("Ex", [896, 1040])
+def test_pcg_long_rows_match_oracle(se, Ex):
+    """Rows of 16 * 896 = 14336 nodes (the longest whose fact table the
+    update kernel copies into LDS) and 16 * 1040 = 16640 (table read from
+    global memory): 40 Jacobi-PCG iterations agree with the oracle's."""
+    _log(f"pcg long rows Ex={Ex}")
+    m, mo = _pair(se, 16, Ex, (False, False), "sinsin", Ey=2)
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(Ex)
+    b = mo.gs(Mo * o.mass(rng.standard_normal(mo.x.shape), mo))
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    dinv = o.jacobi_diag(mo, 1.0, 1.0, Mo)
+    info_o = o.PCGInfo()
+    xo = o.pcg(b, lambda v: o.opLHS(v, 1.0, 1.0, mo, Mo), opM=lambda r: dinv * r,
+               mult=mo.mult, tol=1e-300, maxiter=40, info=info_o)
+    info = se.PCGInfo()
+    with pytest.warns(RuntimeWarning):
+        x = se.to_host(se.pcg(se.to_device(b), op, opM=se.Diagonal(op.jacobi()), mult=m.mult,
+                              tol=1e-300, maxiter=40, info=info))
+    assert info.iters == info_o.iters == 40
+    assert _rel(x, xo) < 1e-10
+
+
 @pytest.mark.parametrize("fma16", ["0", "1"])
 def test_p16_fp64_and_fp32_contraction_paths(se, rows_env, fma16):
     """N = 15: the FP64/FP32 operators default to the exact FP64/FP32 MFMA
