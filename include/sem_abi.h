@@ -235,9 +235,13 @@ typedef struct sem_pcg_info {
  * x0 = 0 (pcg.jl:25; pcg! ignores the incoming x, :71), stop when
  * norm(r, Inf) <= tol (absolute, :36), mult-weighted inner products with the
  * operator mesh's mult (:45, :52), SEM_WARN_MAXITER at k == maxiter (:39).
- * precond: SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG or SEM_PRECOND_DIAG_LOCAL
- * with the device diagonal d (z = d .* r). For SEM_F32 operators b, x and d
- * are float*. x must not alias b. */
+ * tol < 0 selects the relative test norm(r, Inf) <= |tol| * norm(b, Inf),
+ * with norm(b, Inf) formed on the device (no host read before the solve;
+ * time steppers use it). precond: SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG or
+ * SEM_PRECOND_DIAG_LOCAL with the device diagonal d (z = d .* r). For SEM_F32
+ * operators b, x and d are float*. x must not alias b. Returns once the stop
+ * test has fired (info filled in); the last update of x may still be queued
+ * on ctx's stream, so readers of x on other streams synchronise with it. */
 int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d,
             const void* b, void* x, double tol, int maxiter, sem_pcg_info* info);
 
@@ -370,6 +374,12 @@ int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* 
  * (semPS.jl:210-211) */
 int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
             double* result_host);
+/* acc_dev[0] += scale * sum(a .* b .* w) with acc_dev in device memory: the
+ * same sums accumulated over a time loop without a host read per step (the
+ * objective J = sum_n dt/2 sum(B .* u^n .* u^n) of the time-dependent
+ * adjoints, semPS.jl:210-211 per step) */
+int sem_dot_acc(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
+                double scale, double* acc_dev);
 /* y = [QQ'] mask(sum_j coef[j] (B .* if weighted[j]) x[j]): the composition of
  * makeRHS! (convectionDiffusion.jl:93-110, diffusion.jl:51-65): mass terms,
  * explicit history terms, then mask and gatherScatter. nterms <= 8; x[j] are

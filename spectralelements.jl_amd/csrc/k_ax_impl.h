@@ -276,11 +276,19 @@ __global__ void __launch_bounds__(CX* P)
                          &res, &stopped);
     if (stopped) return;
     const int k = prm.pcg.k;
-    const bool done = !(res > prm.pcg.tol) || (k - 1 == prm.pcg.maxiter);
+    // tol < 0: relative stop test norm(r, Inf) <= |tol| norm(b, Inf), where
+    // norm(b, Inf) is iteration 1's res (x0 = 0, r = b); iteration 1 keeps
+    // the absolute value in the state for the later launches
+    double tol = prm.pcg.tol;
+    if (tol < 0.0) {
+      tol = (k == 1) ? -tol * res : prm.pcg.state->tol_eff;
+      if (k == 1 && blockIdx.x == 0 && tid == 0) prm.pcg.state->tol_eff = tol;
+    }
+    const bool done = !(res > tol) || (k - 1 == prm.pcg.maxiter);
     if (done) {
       if (blockIdx.x == 0 && tid == 0) {
         prm.pcg.state->iters = k - 1;
-        prm.pcg.state->converged = res <= prm.pcg.tol ? 1 : 0;
+        prm.pcg.state->converged = res <= tol ? 1 : 0;
         prm.pcg.state->res = res;
         prm.pcg.state->done = 1;
       }

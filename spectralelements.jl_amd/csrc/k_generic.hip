@@ -140,6 +140,7 @@ int pcg_generic(sem_ctx* ctx, const sem_operator* op, const double* d, const dou
   SEM_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   double res = 0.0, t = 0.0, t_prev = 1.0;
   SEM_TRY(norm(r, &res));
+  if (tol < 0.0) tol = -tol * res;   // relative stop test
   int k = 0;
   bool conv = true;
   while (res > tol) {

@@ -268,16 +268,18 @@ k_absmax_part(const double* __restrict__ a, long n, double* part) {
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-__global__ void __launch_bounds__(256) k_final(const double* part, int n, int op, double* out) {
+// op 0: out = sum, 1: out = max (NaN-propagating), 2: out += scale * sum
+__global__ void __launch_bounds__(256) k_final(const double* part, int n, int op, double* out,
+                                               double scale = 1.0) {
   __shared__ double sh[256];
   double acc = 0;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const double v = part[t];
-    if (op == 0) acc += v;
+    if (op != 1) acc += v;
     else acc = (v > acc || v != v) ? v : acc;
   }
-  const double r = op == 0 ? block_reduce_sum(acc, sh) : block_reduce_max(acc, sh);
-  if (threadIdx.x == 0) out[0] = r;
+  const double r = op != 1 ? block_reduce_sum(acc, sh) : block_reduce_max(acc, sh);
+  if (threadIdx.x == 0) out[0] = op == 2 ? out[0] + scale * r : r;
 }
 
 // ---- right-hand-side composition and weighted reductions
@@ -437,9 +439,20 @@ __global__ void k_adjoint_dnu(const double* __restrict__ u, const double* __rest
 //   transpose: the same linear map in T transposed factor by factor,
 //              y = grad'( J'[(J vx) B_D (J w)], J'[(J vy) B_D (J w)] )
 // with grad'(qx, qy) = Dr'(rx qx + ry qy) + (sx qx + sy qy) Ds.
+// The element's fields and the four contraction matrices are staged in LDS
+// once; each phase maps threads to the output points of one contraction and
+// computes every field of that contraction together, so barriers separate
+// phases (5 forward, 6 transpose), not fields. The per-node factors (rx, ry,
+// sx, sy; B_D) are loaded into registers before the first barrier.
 constexpr int kMaxPD = 24;
+constexpr int kAdvThreads = 256;
+static_assert(kMaxP * kMaxP <= kAdvThreads, "one velocity-grid node per thread");
 
-__global__ void __launch_bounds__(256)
+__host__ __device__ constexpr int advect_lds_doubles(int p, int pd) {
+  return 8 * pd * p + 7 * p * p + 2 * pd * pd;
+}
+
+__global__ void __launch_bounds__(kAdvThreads)
 k_advect(const double* __restrict__ T, const double* __restrict__ vx,
          const double* __restrict__ vy, const double* __restrict__ Dr,
          const double* __restrict__ Ds, const double* __restrict__ rx,
@@ -447,144 +460,183 @@ k_advect(const double* __restrict__ T, const double* __restrict__ vx,
          const double* __restrict__ sy, const double* __restrict__ BD,
          const double* __restrict__ Jr, const double* __restrict__ Js, int p, int pd, int Ex,
          int Ey, double* __restrict__ y, int transpose) {
-  // per-element scratch in LDS
-  __shared__ double sT[kMaxP * kMaxP], sA[kMaxP * kMaxP], sB[kMaxP * kMaxP];
-  __shared__ double dA[kMaxPD * kMaxP], dB[kMaxPD * kMaxPD], dC[kMaxPD * kMaxPD],
-      dD[kMaxPD * kMaxPD];
+  extern __shared__ double lds[];
+  const int pp = p * p, dp = pd * p, dd = pd * pd;
+  double* sJr = lds;        // pd x p, column-major
+  double* sJs = sJr + dp;
+  double* sDr = sJs + dp;   // p x p
+  double* sDs = sDr + pp;
+  double* sT = sDs + pp;    // the element's T (forward) or w (transpose)
+  double* sF = sT + pp;     // four velocity-grid fields
+  double* sA = sF + 4 * pp; // four pd x p row interpolants
+  double* sC = sA + 4 * dp; // two dealiased fields
+  double* sE = sC + 2 * dd; // two p x pd back-interpolants
+  const int tid = threadIdx.x;
   const int ex = blockIdx.x % Ex, ey = blockIdx.x / Ex;
-  const int L = p * Ex, LD = pd * Ex;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  auto gi = [&](int a, int b) { return (long)(ex * p + a) + (long)L * (ey * p + b); };
-  auto gd = [&](int a, int b) { return (long)(ex * pd + a) + (long)LD * (ey * pd + b); };
+  const long L = (long)p * Ex, LD = (long)pd * Ex;
+  const long g0 = (long)ex * p + L * ((long)ey * p);
+  const long d0 = (long)ex * pd + LD * ((long)ey * pd);
+  // this thread's velocity-grid node (tid < pp) and dealiased nodes
+  const int a0 = tid % p, b0 = tid / p;
+  const long gn = g0 + a0 + L * b0;
+  const bool own = tid < pp;
+  double frx = 0, fry = 0, fsx = 0, fsy = 0;
+  double bd[3] = {0, 0, 0};
+  if (own) {
+    frx = rx[gn];
+    fry = ry[gn];
+    fsx = sx[gn];
+    fsy = sy[gn];
+    sT[tid] = T[gn];
+    sF[pp + tid] = vx[gn];
+    sF[2 * pp + tid] = vy[gn];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int q = tid + i * kAdvThreads;
+    if (q < dd) bd[i] = BD[d0 + q % pd + LD * (q / pd)];
+  }
+  for (int q = tid; q < dp; q += kAdvThreads) {
+    sJr[q] = Jr[q];
+    sJs[q] = Js[q];
+  }
+  for (int q = tid; q < pp; q += kAdvThreads) {
+    sDr[q] = Dr[q];
+    sDs[q] = Ds[q];
+  }
+  __syncthreads();
   if (!transpose) {
-    for (int q = tid; q < p * p; q += nt) sT[q] = T[gi(q % p, q / p)];
-    __syncthreads();
-    // Tx, Ty on the velocity grid
-    for (int q = tid; q < p * p; q += nt) {
-      const int a = q % p, b = q / p;
+    // F = (Tx, Ty, vx, vy): the gradient of T on the velocity grid
+    if (own) {
       double ur = 0, us = 0;
-      for (int k = 0; k < p; ++k) ur += Dr[a + p * k] * sT[k + p * b];
-      for (int k = 0; k < p; ++k) us += sT[a + p * k] * Ds[b + p * k];
-      const long g = gi(a, b);
-      sA[q] = rx[g] * ur + sx[g] * us;
-      sB[q] = ry[g] * ur + sy[g] * us;
+      for (int k = 0; k < p; ++k) ur += sDr[a0 + p * k] * sT[k + p * b0];
+      for (int k = 0; k < p; ++k) us += sT[a0 + p * k] * sDs[b0 + p * k];
+      sF[tid] = frx * ur + fsx * us;
+      sF[3 * pp + tid] = sF[2 * pp + tid];
+      sF[2 * pp + tid] = sF[pp + tid];
+      sF[pp + tid] = fry * ur + fsy * us;
     }
     __syncthreads();
-    // interpolate four fields to the dealiasing grid: W = Jr F Js'
-    // field 0: Tx, 1: Ty, 2: vx, 3: vy; accumulate c = Jvx JTx + Jvy JTy
-    for (int q = tid; q < pd * pd; q += nt) dD[q] = 0;
+    // rows: A_f = Jr F_f (pd x p)
+    for (int q = tid; q < dp; q += kAdvThreads) {
+      const int ad = q % pd, b = q / pd;
+      double c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      for (int k = 0; k < p; ++k) {
+        const double j = sJr[ad + pd * k];
+        const int f = k + p * b;
+        c0 += j * sF[f];
+        c1 += j * sF[pp + f];
+        c2 += j * sF[2 * pp + f];
+        c3 += j * sF[3 * pp + f];
+      }
+      sA[q] = c0;
+      sA[dp + q] = c1;
+      sA[2 * dp + q] = c2;
+      sA[3 * dp + q] = c3;
+    }
     __syncthreads();
-    for (int f = 0; f < 4; ++f) {
-      for (int q = tid; q < pd * p; q += nt) {  // rows: dA(ad, b) = sum_k Jr(ad,k) F(k,b)
-        const int ad = q % pd, b = q / pd;
-        double acc = 0;
+    // columns: X_f = A_f Js', then c = B_D (X_vx X_Tx + X_vy X_Ty)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int q = tid + i * kAdvThreads;
+      if (q < dd) {
+        const int ad = q % pd, bdd = q / pd;
+        double x0 = 0, x1 = 0, x2 = 0, x3 = 0;
         for (int k = 0; k < p; ++k) {
-          double Fv;
-          if (f == 0) Fv = sA[k + p * b];
-          else if (f == 1) Fv = sB[k + p * b];
-          else if (f == 2) Fv = vx[gi(k, b)];
-          else Fv = vy[gi(k, b)];
-          acc += Jr[ad + pd * k] * Fv;
+          const double j = sJs[bdd + pd * k];
+          const int f = ad + pd * k;
+          x0 += sA[f] * j;
+          x1 += sA[dp + f] * j;
+          x2 += sA[2 * dp + f] * j;
+          x3 += sA[3 * dp + f] * j;
         }
-        dA[q] = acc;
+        sC[q] = (x2 * x0 + x3 * x1) * bd[i];
       }
-      __syncthreads();
-      for (int q = tid; q < pd * pd; q += nt) {
-        const int ad = q % pd, bd = q / pd;
-        double acc = 0;
-        for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
-        if (f == 0) dB[q] = acc;
-        else if (f == 1) dC[q] = acc;
-        else if (f == 2) dD[q] = acc * dB[q];
-        else dD[q] = (dD[q] + acc * dC[q]) * BD[gd(ad, bd)];
-      }
-      __syncthreads();
-    }
-    // back to the velocity grid: y = Jr' C Js
-    for (int q = tid; q < p * pd; q += nt) {
-      const int a = q % p, bd = q / p;
-      double acc = 0;
-      for (int k = 0; k < pd; ++k) acc += Jr[k + pd * a] * dD[k + pd * bd];
-      dA[q] = acc;  // (p x pd)
     }
     __syncthreads();
-    for (int q = tid; q < p * p; q += nt) {
-      const int a = q % p, b = q / p;
+    // back, rows: E = Jr' C (p x pd)
+    for (int q = tid; q < dp; q += kAdvThreads) {
+      const int a = q % p, bdd = q / p;
+      double e = 0;
+      for (int k = 0; k < pd; ++k) e += sJr[k + pd * a] * sC[k + pd * bdd];
+      sE[q] = e;
+    }
+    __syncthreads();
+    // back, columns: y = E Js
+    if (own) {
       double acc = 0;
-      for (int k = 0; k < pd; ++k) acc += dA[a + p * k] * Js[k + pd * b];
-      y[gi(a, b)] = acc;
+      for (int k = 0; k < pd; ++k) acc += sE[a0 + p * k] * sJs[k + pd * b0];
+      y[gn] = acc;
     }
   } else {
-    // transpose with respect to T: y = grad'( J'[B_D Jvx (J w)], J'[B_D Jvy (J w)] )
-    for (int q = tid; q < p * p; q += nt) sT[q] = T[gi(q % p, q / p)];  // T holds w
+    // F = (w, vx, vy) with w in sT
+    if (own) sF[tid] = sT[tid];
     __syncthreads();
-    // c = B_D .* (J w)
-    for (int q = tid; q < pd * p; q += nt) {
+    for (int q = tid; q < dp; q += kAdvThreads) {
       const int ad = q % pd, b = q / pd;
-      double acc = 0;
-      for (int k = 0; k < p; ++k) acc += Jr[ad + pd * k] * sT[k + p * b];
-      dA[q] = acc;
+      double c0 = 0, c1 = 0, c2 = 0;
+      for (int k = 0; k < p; ++k) {
+        const double j = sJr[ad + pd * k];
+        const int f = k + p * b;
+        c0 += j * sF[f];
+        c1 += j * sF[pp + f];
+        c2 += j * sF[2 * pp + f];
+      }
+      sA[q] = c0;
+      sA[dp + q] = c1;
+      sA[2 * dp + q] = c2;
     }
     __syncthreads();
-    for (int q = tid; q < pd * pd; q += nt) {
-      const int ad = q % pd, bd = q / pd;
-      double acc = 0;
-      for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
-      dD[q] = acc * BD[gd(ad, bd)];
+    // c = B_D (J w); C_0 = (J vx) c, C_1 = (J vy) c
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int q = tid + i * kAdvThreads;
+      if (q < dd) {
+        const int ad = q % pd, bdd = q / pd;
+        double x0 = 0, x1 = 0, x2 = 0;
+        for (int k = 0; k < p; ++k) {
+          const double j = sJs[bdd + pd * k];
+          const int f = ad + pd * k;
+          x0 += sA[f] * j;
+          x1 += sA[dp + f] * j;
+          x2 += sA[2 * dp + f] * j;
+        }
+        const double c = x0 * bd[i];
+        sC[q] = x1 * c;
+        sC[dd + q] = x2 * c;
+      }
     }
     __syncthreads();
-    // J vx, J vy on the dealiasing grid -> dB, dC, then products with c
-    for (int f = 0; f < 2; ++f) {
-      for (int q = tid; q < pd * p; q += nt) {
-        const int ad = q % pd, b = q / pd;
-        double acc = 0;
-        for (int k = 0; k < p; ++k) acc += Jr[ad + pd * k] * (f == 0 ? vx[gi(k, b)] : vy[gi(k, b)]);
-        dA[q] = acc;
+    for (int q = tid; q < dp; q += kAdvThreads) {
+      const int a = q % p, bdd = q / p;
+      double e0 = 0, e1 = 0;
+      for (int k = 0; k < pd; ++k) {
+        const double j = sJr[k + pd * a];
+        e0 += j * sC[k + pd * bdd];
+        e1 += j * sC[dd + k + pd * bdd];
       }
-      __syncthreads();
-      for (int q = tid; q < pd * pd; q += nt) {
-        const int ad = q % pd, bd = q / pd;
-        double acc = 0;
-        for (int k = 0; k < p; ++k) acc += dA[ad + pd * k] * Js[bd + pd * k];
-        if (f == 0) dB[q] = acc * dD[q];
-        else dC[q] = acc * dD[q];
-      }
-      __syncthreads();
+      sE[q] = e0;
+      sE[dp + q] = e1;
     }
-    // back-interpolate both: sA = J' dB, sB = J' dC
-    for (int f = 0; f < 2; ++f) {
-      const double* src = f == 0 ? dB : dC;
-      for (int q = tid; q < p * pd; q += nt) {
-        const int a = q % p, bd = q / p;
-        double acc = 0;
-        for (int k = 0; k < pd; ++k) acc += Jr[k + pd * a] * src[k + pd * bd];
-        dA[q] = acc;
+    __syncthreads();
+    // S_f = E_f Js on this thread's node, then the grad' operands
+    // P1 = rx S0 + ry S1, P2 = sx S0 + sy S1 (reusing sF)
+    if (own) {
+      double s0 = 0, s1 = 0;
+      for (int k = 0; k < pd; ++k) {
+        const double j = sJs[k + pd * b0];
+        s0 += sE[a0 + p * k] * j;
+        s1 += sE[dp + a0 + p * k] * j;
       }
-      __syncthreads();
-      for (int q = tid; q < p * p; q += nt) {
-        const int a = q % p, b = q / p;
-        double acc = 0;
-        for (int k = 0; k < pd; ++k) acc += dA[a + p * k] * Js[k + pd * b];
-        (f == 0 ? sA : sB)[q] = acc;
-      }
-      __syncthreads();
+      sF[tid] = frx * s0 + fry * s1;
+      sF[pp + tid] = fsx * s0 + fsy * s1;
     }
-    // grad'(sA, sB) = Dr'(rx sA + ry sB) + (sx sA + sy sB) Ds
-    for (int q = tid; q < p * p; q += nt) {
-      const int a = q % p, b = q / p;
-      double acc = 0;
-      for (int k = 0; k < p; ++k) {
-        const long g = gi(k, b);
-        const int qq = k + p * b;
-        acc += Dr[k + p * a] * (rx[g] * sA[qq] + ry[g] * sB[qq]);
-      }
-      double acc2 = 0;
-      for (int k = 0; k < p; ++k) {
-        const long g = gi(a, k);
-        const int qq = a + p * k;
-        acc2 += (sx[g] * sA[qq] + sy[g] * sB[qq]) * Ds[k + p * b];
-      }
-      y[gi(a, b)] = acc + acc2;
+    __syncthreads();
+    if (own) {
+      double acc = 0, acc2 = 0;
+      for (int k = 0; k < p; ++k) acc += sDr[k + p * a0] * sF[k + p * b0];
+      for (int k = 0; k < p; ++k) acc2 += sF[pp + a0 + p * k] * sDs[k + p * b0];
+      y[gn] = acc + acc2;
     }
   }
 }
@@ -733,6 +785,14 @@ int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, cons
   return SEM_OK;
 }
 
+int launch_dot_acc(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
+                   double scale, double* part, int nparts, double* acc) {
+  hipLaunchKernelGGL(k_dot_part, dim3(nparts), dim3(256), 0, st, a, b, w, (long)n, part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 2, acc, scale);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
 int launch_gprod(hipStream_t st, int64_t n, const double* G11, const double* G12,
                  const double* G22, const double* a, const double* b, double* wr, double* ws) {
   hipLaunchKernelGGL(k_gprod, dim3(nblk(n)), dim3(256), 0, st, G11, G12, G22, a, b, wr, ws,
@@ -807,7 +867,9 @@ int launch_advect(hipStream_t st, const sem_mesh* mV, const sem_mesh* mD, const 
   const int E = mV->Ex * mV->Ey;
   double* const* f = mV->f;
   // J holds Jr (pd x p) followed by Js (pd x p), column-major
-  hipLaunchKernelGGL(k_advect, dim3(E), dim3(256), 0, st, T, vx, vy, mV->Dr, mV->Ds, f[SEM_RX],
+  const size_t lds = sizeof(double) * advect_lds_doubles(p, pd);
+  hipLaunchKernelGGL(k_advect, dim3(E), dim3(kAdvThreads), lds, st, T, vx, vy, mV->Dr, mV->Ds,
+                     f[SEM_RX],
                      f[SEM_RY], f[SEM_SX], f[SEM_SY], mD->f[SEM_B], J, J + pd * p, p, pd, mV->Ex,
                      mV->Ey, y, transpose);
   SEM_HIP(hipGetLastError());

@@ -28,17 +28,21 @@
 namespace sem {
 namespace {
 
+// init and partials passes: one block per partial, 1024 threads, 32-bit
+// node indices (meshes hold fewer than 2^31 local nodes)
+constexpr int kInitThreads = 1024;
+
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kInitThreads)
 k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
            const T* __restrict__ d, int P, int L, int NC, int per_x, int per_y, long n,
            double* __restrict__ part, int n3) {
-  __shared__ double red[256];
+  __shared__ double red[kInitThreads];
   double rho = 0.0, mx = 0.0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % L);
-    const int j = (int)(t / L);
+  const int nn = (int)n;
+  for (int t = blockIdx.x * kInitThreads + threadIdx.x; t < nn; t += gridDim.x * kInitThreads) {
+    const int i = t % L;
+    const int j = t / L;
     const T bv = b[t];
     x[t] = T(0);
     r[t] = bv;
@@ -47,8 +51,8 @@ k_pcg_init(const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __r
     rho += (double)bv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
     mx = nanmax(mx, fabs((double)bv));
   }
-  rho = block_sum(rho, red, 256);
-  mx = block_max(mx, red, 256);
+  rho = block_sum(rho, red, kInitThreads);
+  mx = block_max(mx, red, kInitThreads);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = mx;
@@ -375,22 +379,22 @@ inline int nblk(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 // Partials of <r, h> and norm(r, Inf) of the current residual (after a
 // residual replacement).
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kInitThreads)
 k_pcg_partials(const T* __restrict__ r, const T* __restrict__ d, int P, int L, int NC, int per_x,
                int per_y, long n, double* __restrict__ part, int n3) {
-  __shared__ double red[256];
+  __shared__ double red[kInitThreads];
   double rho = 0.0, mx = 0.0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % L);
-    const int j = (int)(t / L);
+  const int nn = (int)n;
+  for (int t = blockIdx.x * kInitThreads + threadIdx.x; t < nn; t += gridDim.x * kInitThreads) {
+    const int i = t % L;
+    const int j = t / L;
     const T rv = r[t];
     const T z = d ? d[t] * rv : rv;
     rho += (double)rv * (double)z * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
     mx = nanmax(mx, fabs((double)rv));
   }
-  rho = block_sum(rho, red, 256);
-  mx = block_max(mx, red, 256);
+  rho = block_sum(rho, red, kInitThreads);
+  mx = block_max(mx, red, kInitThreads);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = rho;
     part[n3 + blockIdx.x] = mx;
@@ -477,7 +481,8 @@ struct PcgRun {
   int init(const T* b, T* x) {
     hipStream_t st = ctx->stream;
     SEM_HIP(hipMemsetAsync(state, 0, sizeof(PcgState), st));
-    hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(256), 0, st, b, x, r, p, d, g.P, g.L,
+    hipLaunchKernelGGL((k_pcg_init<T>), dim3(n3), dim3(kInitThreads), 0, st, b, x, r, p, d, g.P,
+                       g.L,
                        g.NC, g.per_x, g.per_y, n, part, n3);
     SEM_HIP(hipGetLastError());
     return SEM_OK;
@@ -487,7 +492,8 @@ struct PcgRun {
   int reprime(int k) {
     hipStream_t st = ctx->stream;
     SEM_HIP(hipMemsetAsync(&state->done, 0, sizeof(int), st));
-    hipLaunchKernelGGL((k_pcg_partials<T>), dim3(n3), dim3(256), 0, st, r, d, g.P, g.L, g.NC,
+    hipLaunchKernelGGL((k_pcg_partials<T>), dim3(n3), dim3(kInitThreads), 0, st, r, d, g.P, g.L,
+                       g.NC,
                        g.per_x, g.per_y, n, part + 2 * n3 * ((k - 1) & 1), n3);
     SEM_HIP(hipGetLastError());
     return SEM_OK;
@@ -518,8 +524,11 @@ struct PcgRun {
   // launches iterations k_start.. until the stop test fires (norm(r, Inf) <=
   // tol or k - 1 == maxiter); on return hs holds the final state
   // hint: a fresh solve starts with a batch the size of this operator's last
-  // converged solve (repeated solves: time steps, adjoints, FD) and records it
-  int run(T* x, double tol, int maxiter, int k_start, bool hint = false) {
+  // converged solve (repeated solves: time steps, adjoints, FD) and records it.
+  // ev_end (optional) is recorded before each state read, so it has completed
+  // when this returns; the last update of x is left queued on the stream.
+  int run(T* x, double tol, int maxiter, int k_start, bool hint = false,
+          hipEvent_t ev_end = nullptr) {
     hipStream_t st = ctx->stream;
     int k = k_start,
         batch = (hint && op->last_iters > 0) ? std::min(op->last_iters + 2, 1024) : 4;
@@ -542,6 +551,7 @@ struct PcgRun {
         launch_update(st, k);
       }
       SEM_HIP(hipGetLastError());
+      if (ev_end) SEM_HIP(hipEventRecord(ev_end, st));
       SEM_HIP(hipMemcpyAsync(hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
       SEM_HIP(hipStreamSynchronize(st));
       if (hs->done) {
@@ -579,9 +589,9 @@ int pcg_solve(PcgRun<T>& R, sem_ctx* ctx, const sem_operator* op, const T* b, T*
   hipStream_t st = ctx->stream;
   SEM_HIP(hipEventRecord(ctx->ev0, st));
   SEM_TRY(R.init(b, x));
-  SEM_TRY(R.run(x, tol, maxiter, 1, true));
-  SEM_HIP(hipEventRecord(ctx->ev1, st));
-  SEM_HIP(hipEventSynchronize(ctx->ev1));
+  // ev1 completes inside run (before its last state read); the final x
+  // update stays queued on the stream, so no second wait here
+  SEM_TRY(R.run(x, tol, maxiter, 1, true, ctx->ev1));
   float ms = 0.f;
   SEM_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   const PcgState* hs = R.hs;
@@ -935,7 +945,7 @@ extern "C" int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond
           "sem_pcg_noalloc: SEM_PRECOND_DIAG needs the diagonal d");
   SEM_ARG(x != b, "sem_pcg_noalloc: x must not alias b");
   SEM_ARG(maxiter >= 0, "sem_pcg_noalloc: maxiter < 0");
-  SEM_ARG(tol >= 0.0, "sem_pcg_noalloc: tol < 0");
+  SEM_ARG(tol == tol, "sem_pcg_noalloc: tol is NaN");
   if (precond == SEM_PRECOND_IDENTITY) d = nullptr;
   if (op->generic) {
     set_error("sem_pcg_noalloc: meshes with nr != ns or nr > %d take sem_pcg", kMaxP);

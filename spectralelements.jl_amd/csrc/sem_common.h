@@ -178,6 +178,8 @@ struct PcgState {
   // alpha of iteration k at [k & 1]: x += alpha p is applied by the next
   // iteration's apply, and by a finishing pass after the last one
   double alpha[2];
+  // relative stop test (tol < 0): |tol| * norm(b, Inf), set by iteration 1
+  double tol_eff;
 };
 
 struct PcgDev {
@@ -252,6 +254,9 @@ int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* 
                 int nparts, double* out);
 int launch_norm_inf(hipStream_t st, int64_t n, const double* a, double* part, int nparts,
                     double* out);
+// acc[0] += scale * sum(a .* b .* w) (w may be NULL), acc in device memory
+int launch_dot_acc(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
+                   double scale, double* part, int nparts, double* acc);
 
 int launch_grad(hipStream_t st, const sem_mesh* m, const double* u, double* ux, double* uy);
 int launch_grad_t(hipStream_t st, const sem_mesh* m, const double* vx, const double* vy,

@@ -695,7 +695,7 @@ int sem_pcg(sem_ctx* ctx, const sem_operator* op, int precond, const void* d, co
   SEM_ARG(precond != SEM_PRECOND_DIAG_LOCAL || op->precision == SEM_F64,
           "sem_pcg: SEM_PRECOND_DIAG_LOCAL needs an SEM_F64 operator");
   SEM_ARG(maxiter >= 0, "sem_pcg: maxiter < 0");
-  SEM_ARG(tol >= 0.0, "sem_pcg: tol < 0");
+  SEM_ARG(tol == tol, "sem_pcg: tol is NaN");
   return pcg_run(ctx, op, precond, d, b, x, tol, maxiter, info);
 }
 
@@ -723,6 +723,14 @@ int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const dou
   SEM_HIP(hipMemcpyAsync(result, part + np, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   SEM_HIP(hipStreamSynchronize(ctx->stream));
   return SEM_OK;
+}
+
+int sem_dot_acc(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
+                double scale, double* acc) {
+  SEM_ARG(ctx && a && b && acc && n >= 0, "bad arguments");
+  const int np = 256;
+  SEM_TRY(ensure_ws(ctx, sizeof(double) * np));
+  return launch_dot_acc(ctx->stream, n, a, b, w, scale, static_cast<double*>(ctx->ws), np, acc);
 }
 
 int sem_combine(sem_ctx* ctx, const sem_mesh* m, int nterms, const double* coef,

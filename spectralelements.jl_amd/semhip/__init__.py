@@ -11,7 +11,7 @@ from .core import (  # noqa: F401
     gordonHall, gordonHallMap, semInterp,
     deform_sinsin, derivMat, empty_field, fixU, gatherScatter, gausslobatto, generateMask, grad,
     gradT, hlmz, identity, interpMat, lapl, mask, mass, ndgrid, norm_inf, pcg, pcg_, pcg_mixed,
-    semmesh, synchronize, to_device, to_host, wdot, dot, combine, vardiff_factors, annulus,
+    semmesh, synchronize, to_device, to_host, wdot, dot, dot_acc, combine, vardiff_factors, annulus,
 )
 from .adjoint import (  # noqa: F401
     VarDiffusion, VarDiffusionOperator, diffusion_solver, linsolve, nodal_gradient,

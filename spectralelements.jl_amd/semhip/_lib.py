@@ -108,6 +108,7 @@ SIGNATURES = {
     "sem_wdot": [_vp, _vp, _vp, _vp, _dp],
     "sem_norm_inf": [_vp, _i64, _vp, _dp],
     "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],
+    "sem_dot_acc": [_vp, _i64, _vp, _vp, _vp, C.c_double, _vp],
     "sem_combine": [_vp, _vp, _i, _dp, _pvp, C.POINTER(C.c_int), _i, _i, _vp],
     "sem_vardiff_factors": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sem_gprod": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -36,8 +36,8 @@ from typing import Callable, List, Optional
 import numpy as np
 import torch
 
-from .core import (Diagonal, Mesh, PCGInfo, SEMOperator, advect, combine, derivMat, dot,
-                   empty_field, generateMask, interpMat, lapl, norm_inf, pcg, to_device)
+from .core import (Diagonal, Mesh, PCGInfo, SEMOperator, advect, combine, derivMat, dot, dot_acc,
+                   empty_field, generateMask, interpMat, lapl, pcg, to_device)
 
 
 # ------------------------------------------------------------ time stepper
@@ -196,9 +196,10 @@ class ConvectionDiffusion:
     def solve_(self):
         """pcg!(u, rhs, opLHS; opM, mult) then u .+= ub (convectionDiffusion.jl:112-122)."""
         op, d = self._op(float(self.tstep.bdfB[0]))
-        tol = self.abs_tol if self.abs_tol is not None else self.tol * norm_inf(self.rhs)
-        x = pcg(self.rhs, op, opM=d, mult=self.mshV.mult, tol=tol, maxiter=self.maxiter,
-                info=self.info, x=self.fld.u)
+        # relative tolerance: norm(rhs, Inf) is formed on the device
+        kw = {"tol": self.abs_tol} if self.abs_tol is not None else {"rtol": self.tol}
+        x = pcg(self.rhs, op, opM=d, mult=self.mshV.mult, maxiter=self.maxiter,
+                info=self.info, x=self.fld.u, **kw)
         self.iters_total += self.info.iters
         if not self.fld._ub_zero:
             x.add_(self.fld.ub)
@@ -282,9 +283,12 @@ class CDGradient:
         self.iters_fwd, self.iters_adj = iters_fwd, iters_adj
 
 
-def cd_objective_step(cdn: ConvectionDiffusion) -> float:
-    """dt/2 sum_local B u^2 of the current state."""
+def cd_objective_step(cdn: ConvectionDiffusion, acc: Optional[torch.Tensor] = None):
+    """dt/2 sum_local B u^2 of the current state; with acc (a one-element
+    device tensor) it is added there on the device instead of returned."""
     u = cdn.fld.u
+    if acc is not None:
+        return dot_acc(acc, u, u, cdn.mshV.B, 0.5 * cdn.tstep.dt)
     return 0.5 * cdn.tstep.dt * dot(u, u, cdn.mshV.B)
 
 
@@ -316,13 +320,14 @@ def cd_gradient(cdn: ConvectionDiffusion, u0, nsteps: int,
     cdn.iters_total = 0
     snaps = {}
     coefs = {}
-    J = 0.0
+    Jd = torch.zeros(1, dtype=torch.float64, device=cdn.fld.u.device)
     for n in range(1, N + 1):
         if (n - 1) % c == 0:
             snaps[n - 1] = cdn.snapshot()
         cdn.step_()
         coefs[n] = (cdn.tstep.bdfA.copy(), cdn.tstep.bdfB.copy())
-        J += cd_objective_step(cdn)
+        cd_objective_step(cdn, Jd)
+    J = float(Jd)
     iters_fwd = cdn.iters_total
     steps = N
     acc = {}
@@ -345,8 +350,8 @@ def cd_gradient(cdn: ConvectionDiffusion, u0, nsteps: int,
                 terms.append((1.0, acc.pop(n), False))
             mu = combine(mV, terms, M=M, assemble=True)
             op, d = cdn._op(float(b[0]))
-            tol = cdn.abs_tol if cdn.abs_tol is not None else cdn.tol * norm_inf(mu)
-            lam = pcg(mu, op, opM=d, mult=mV.mult, tol=tol, maxiter=cdn.maxiter, info=info)
+            kw = {"tol": cdn.abs_tol} if cdn.abs_tol is not None else {"rtol": cdn.tol}
+            lam = pcg(mu, op, opM=d, mult=mV.mult, maxiter=cdn.maxiter, info=info, **kw)
             iters_adj += info.iters
             Ct = None
             for i in range(k):
@@ -374,8 +379,8 @@ def cd_gradient(cdn: ConvectionDiffusion, u0, nsteps: int,
 def cd_objective(cdn: ConvectionDiffusion, u0, nsteps: int) -> float:
     """J of ``cd_gradient`` by a plain forward run from u0."""
     cdn.reset_(u0)
-    J = 0.0
+    Jd = torch.zeros(1, dtype=torch.float64, device=cdn.fld.u.device)
     for _ in range(int(nsteps)):
         cdn.step_()
-        J += cd_objective_step(cdn)
-    return J
+        cd_objective_step(cdn, Jd)
+    return float(Jd)

@@ -591,6 +591,15 @@ def wdot(a, b, msh: Mesh) -> float:
     return r.value
 
 
+def dot_acc(acc: torch.Tensor, a, b, w=None, scale: float = 1.0) -> torch.Tensor:
+    """acc += scale * sum(a .* b .* w) on the device (acc: a one-element FP64
+    device tensor), without a host read: sums accumulated over a time loop."""
+    a, b = to_device(a), to_device(b)
+    wd = to_device(w) if w is not None else None
+    L.call("sem_dot_acc", ctx(), a.numel(), _p(a), _p(b), _p(wd), float(scale), _p(acc))
+    return acc
+
+
 def dot(a, b, w=None) -> float:
     """sum(a .* b .* w) (w None: sum(a .* b)) over all local nodes."""
     a, b = to_device(a), to_device(b)
@@ -796,15 +805,22 @@ identity._sem_identity = True
 
 
 def pcg(b, opA, opM=None, mult=None, ifv: bool = False, tol: float = 1e-8,
-        maxiter: Optional[int] = None, info: Optional[PCGInfo] = None, x=None):
+        maxiter: Optional[int] = None, info: Optional[PCGInfo] = None, x=None,
+        rtol: Optional[float] = None):
     """Left-preconditioned CG from x0 = 0 (pcg.jl:16-60): stops when
     norm(r, Inf) <= tol, inner products weighted by mult (None: unweighted,
     the reference's mult = ones), warns and returns at k == maxiter. An
     SEMOperator with the operator mesh's mult and an identity, Diagonal or MG
     opM runs fused on the GPU; anything else runs the same recursion with
-    device vector algebra."""
+    device vector algebra. rtol (instead of tol): stop when norm(r, Inf) <=
+    rtol * norm(b, Inf); the fused Jacobi/identity solve forms norm(b, Inf) on
+    the device, without a host read."""
     was_np = isinstance(b, np.ndarray)
     info = info if info is not None else PCGInfo()
+    fused_diag = isinstance(opA, SEMOperator) and _same_mult(mult, opA.msh) and (
+        _is_identity(opM) or isinstance(opM, Diagonal))
+    if rtol is not None:
+        tol = -float(rtol) if fused_diag else float(rtol) * norm_inf(b)
     if isinstance(opA, SEMOperator) and isinstance(opM, MG) and opM.op is opA and (
             _same_mult(mult, opA.msh)):
         bd = to_device(b)
@@ -821,8 +837,7 @@ def pcg(b, opA, opM=None, mult=None, ifv: bool = False, tol: float = 1e-8,
         if ifv:
             print(f"PCG iter: {pi.iters}, res: {pi.res}")
         return _wrap_out(xd, was_np)
-    if isinstance(opA, SEMOperator) and _same_mult(mult, opA.msh) and (
-            _is_identity(opM) or isinstance(opM, Diagonal)):
+    if fused_diag:
         bd = to_device(b, dtype=opA.dtype)
         if maxiter is None:
             maxiter = bd.numel()

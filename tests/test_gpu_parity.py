@@ -267,6 +267,40 @@ def test_pcg_jacobi_helmholtz_matches_oracle(se, per, bc):
         assert _rel(u, uo) < 10 * rel
 
 
+def test_pcg_relative_tolerance_on_device_and_dot_acc(se):
+    """rtol: the fused solve's stop test norm(r, Inf) <= rtol norm(b, Inf)
+    (norm(b, Inf) formed on the device) runs the same iterations as the
+    absolute tol = rtol norm(b, Inf); the generic path and the host-driven
+    path agree; sem_dot_acc accumulates sum(a .* b .* w) on the device."""
+    import torch
+    _log("pcg rtol / dot_acc")
+    m, mo = _pair(se, 8, 12, (False, False), "sinsin", 10)
+    M = se.generateMask("DDDD", m)
+    Mo = o.generateMask("DDDD", mo)
+    rng = np.random.default_rng(8)
+    b = mo.gs(Mo * o.mass(rng.standard_normal(mo.x.shape), mo))
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    d = se.Diagonal(op.jacobi())
+    tol = 1e-9 * np.abs(b).max()
+    ia, ir, ig = se.PCGInfo(), se.PCGInfo(), se.PCGInfo()
+    xa = se.to_host(se.pcg(se.to_device(b), op, opM=d, mult=m.mult, tol=tol, info=ia))
+    xr = se.to_host(se.pcg(se.to_device(b), op, opM=d, mult=m.mult, rtol=1e-9, info=ir))
+    xg = se.to_host(se.pcg(se.to_device(b), lambda v: op(v), opM=d, mult=m.mult, rtol=1e-9,
+                           info=ig))
+    assert ia.iters == ir.iters and ir.converged and ir.res <= tol
+    np.testing.assert_array_equal(xa, xr)
+    assert abs(ig.iters - ir.iters) <= 2 and _rel(xg, xr) < 1e-8
+    zero = se.PCGInfo()
+    se.pcg(se.to_device(0 * b), op, opM=d, mult=m.mult, rtol=1e-9, info=zero)
+    assert zero.iters == 0 and zero.converged
+    acc = torch.zeros(1, dtype=torch.float64, device="cuda")
+    u, v = rng.standard_normal(mo.x.shape), rng.standard_normal(mo.x.shape)
+    se.dot_acc(acc, se.to_device(u), se.to_device(v), m.B, 0.5)
+    se.dot_acc(acc, se.to_device(u), se.to_device(u), None, 2.0)
+    ref = 0.5 * np.sum(u * v * mo.B) + 2.0 * np.sum(u * u)
+    assert abs(float(acc) - ref) <= 1e-12 * np.sum(np.abs(u * v * mo.B) + 2 * u * u)
+
+
 def test_pcg_maxiter_and_zero_rhs(se):
     _log("pcg maxiter / zero rhs")
     m, mo, M, Mo, b = _cfg1(se)
