@@ -107,6 +107,12 @@ int sem_gausslobatto(int n, double* z_host, double* w_host);
 int sem_derivmat(int n, const double* x_host, double* D_host);
 /* interpMat(xo, xi), interp.jl:10-35; J_host is no*ni column-major */
 int sem_interpmat(int no, const double* xo_host, int ni, const double* xi_host, double* J_host);
+/* bdfExtK(t, k), time.jl:31-53: the BDFk/EXTk coefficients of the time
+ * history t[0..nt) = [t_{n+1}, t_n, ...]: b = first row of derivMat(unique(t)),
+ * a = interpMat(t_{n+1}, unique(t)[1:]), zero-padded to k (a) and k + 1 (b)
+ * while the history holds fewer distinct times, truncated otherwise; a single
+ * distinct time gives a = [1, 0, ...]. a_host has k entries, b_host k + 1. */
+int sem_bdfextk(int nt, const double* t_host, int k, double* a_host, double* b_host);
 /* semmesh(E, n), semmesh.jl:9-27: z_host, w_host have E*n entries */
 int sem_semmesh(int E, int n, double* z_host, double* w_host);
 
@@ -415,6 +421,36 @@ int sem_advect(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD,
 /* transpose of advect with respect to T (for the discrete adjoint) */
 int sem_advect_t(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD,
                  const double* w, const double* vx, const double* vy, double* y);
+
+/* ------------------------------------- convection-diffusion time loop */
+/* The step loop of ConvectionDiffusion (convectionDiffusion.jl:5-179) run by
+ * the library: sem_cd_steps repeats step!(cdn) (:148-155) nsteps times with
+ * f = 0 and ub = 0 (homogeneous Dirichlet data on the bc_bits sides):
+ * updateHist! of the field and the TimeStepper (mesh.jl:199-215,
+ * time.jl:101-112), bdfExtK (time.jl:31-53), makeRHS! (:93-110, advect of a
+ * history computed once and shifted with it), solve! (:112-122) as the fused
+ * Jacobi PCG of mask(QQ'(hlmz(., nu, bdfB[1]))) with the relative stop test
+ * norm(r, Inf) <= tol * norm(rhs, Inf). One operator and Jacobi diagonal per
+ * distinct bdfB[1] is built on first use and kept. nu: scalar, or the field
+ * nu_f (device, kept by the caller). maxiter 0: the number of local nodes
+ * (pcg.jl:22). Order k in [1, 4]. */
+typedef struct sem_cd sem_cd;
+int sem_cd_create(sem_ctx* ctx, const sem_mesh* meshV, const sem_mesh* meshD, int bc_bits,
+                  int k, double nu, const double* nu_f, const double* vx, const double* vy,
+                  double dt, double tol, int maxiter, sem_cd** out);
+int sem_cd_destroy(sem_cd* cd);
+/* State (caller-owned device fields, host arrays updated in place): u;
+ * uh[k] the histories and adv[k] the cached advect(uh_i) buffers (the
+ * pointer arrays rotate like the histories), adv_valid[k] whether adv[i]
+ * holds advect(uh[i]); t[k+1] the TimeStepper times. Optional outputs:
+ * J_dev[0] += dt/2 sum(B .* u .* u) after every step (device); a_out
+ * (nsteps x k) and b_out (nsteps x (k+1)) the per-step coefficients;
+ * iters_total the PCG iterations summed. SEM_WARN_MAXITER if some solve hit
+ * maxiter. The steps issue the same launches as the step-by-step entry
+ * points, in the same order (identical results). */
+int sem_cd_steps(sem_ctx* ctx, sem_cd* cd, int nsteps, double* u, double** uh, double** adv,
+                 int* adv_valid, double* t_host, double* J_dev, double* a_out_host,
+                 double* b_out_host, int64_t* iters_total);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ LIB = os.path.join(LIBDIR, "libsemhip.so")
 UNITS = ([("sem_host.cpp", [], "sem_host"), ("k_ax.hip", [], "k_ax"),
           ("k_misc.hip", [], "k_misc"), ("k_pcg.hip", [], "k_pcg"),
           ("k_mg.hip", [], "k_mg"), ("k_geom.hip", [], "k_geom"),
-          ("k_generic.hip", [], "k_generic")]
+          ("k_generic.hip", [], "k_generic"), ("sem_cd.cpp", [], "sem_cd")]
          + [("k_ax_inst.hip", [f"-DSEM_P={p}", f"-DSEM_PART={q}"], f"k_ax_p{p}_{q}")
             for p in range(16, 1, -1) for q in (range(7) if p == 16 else range(2))])
 

@@ -278,6 +278,21 @@ int sem_interpmat(int no, const double* xo, int ni, const double* xi, double* J)
   return SEM_OK;
 }
 
+int sem_bdfextk(int nt, const double* t, int k, double* a, double* b) {
+  SEM_ARG(nt >= 1 && k >= 1 && t && a && b, "sem_bdfextk: bad arguments");
+  std::vector<double> u;
+  for (int i = 0; i < nt; ++i)
+    if (std::find(u.begin(), u.end(), t[i]) == u.end()) u.push_back(t[i]);
+  const int m = (int)u.size(), kk = m - 1;
+  std::vector<double> D((size_t)m * m), J(std::max(kk, 1));
+  derivmat(m, u.data(), D.data());
+  if (kk > 0) interpmat(1, u.data(), kk, u.data() + 1, J.data());
+  for (int i = 0; i < k; ++i) a[i] = i < kk ? J[i] : 0.0;
+  for (int i = 0; i <= k; ++i) b[i] = i < m ? D[(size_t)m * i] : 0.0;
+  if (kk == 0) a[0] = 1.0;
+  return SEM_OK;
+}
+
 int sem_semmesh(int E, int n, double* z, double* w) {
   SEM_ARG(E >= 1 && n >= 2 && z, "sem_semmesh: bad arguments");
   semmesh(E, n, z, w);

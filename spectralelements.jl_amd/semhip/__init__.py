@@ -18,6 +18,7 @@ from .adjoint import (  # noqa: F401
 )
 from .cd import (  # noqa: F401
     CDGradient, ConvectionDiffusion, Field, TimeStepper, bdfExtK, cd_gradient, cd_objective,
+    cd_objective_step,
 )
 from .diffusion import Diffusion  # noqa: F401
 from . import workloads  # noqa: F401

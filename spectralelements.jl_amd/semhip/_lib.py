@@ -108,6 +108,11 @@ SIGNATURES = {
     "sem_wdot": [_vp, _vp, _vp, _vp, _dp],
     "sem_norm_inf": [_vp, _i64, _vp, _dp],
     "sem_dot": [_vp, _i64, _vp, _vp, _vp, _dp],
+    "sem_bdfextk": [_i, _vp, _i, _vp, _vp],
+    "sem_cd_create": [_vp, _vp, _vp, _i, _i, _d, _vp, _vp, _vp, _d, _d, _i, _pvp],
+    "sem_cd_destroy": [_vp],
+    "sem_cd_steps": [_vp, _vp, _i, _vp, _pvp, _pvp, C.POINTER(C.c_int), _vp, _vp, _vp, _vp,
+                     C.POINTER(C.c_int64)],
     "sem_dot_acc": [_vp, _i64, _vp, _vp, _vp, C.c_double, _vp],
     "sem_combine": [_vp, _vp, _i, _dp, _pvp, C.POINTER(C.c_int), _i, _i, _vp],
     "sem_vardiff_factors": [_vp, _vp, _vp, _vp, _vp, _vp],
@@ -130,6 +135,8 @@ def load(path: str = None):
     environment variable SEMHIP_LIB) and declares every entry point; raises
     if it is missing."""
     global _lib
+    if _lib is not None and path is None:
+        return _lib
     path = path or os.environ.get("SEMHIP_LIB") or LIB_PATH
     with _lock:
         if _lib is not None:

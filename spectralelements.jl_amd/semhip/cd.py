@@ -30,41 +30,30 @@ Differences from the reference, all value-preserving or opt-in:
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
+import weakref
 from typing import Callable, List, Optional
 
 import numpy as np
 import torch
 
-from .core import (Diagonal, Mesh, PCGInfo, SEMOperator, advect, combine, derivMat, dot, dot_acc,
-                   empty_field, generateMask, interpMat, lapl, pcg, to_device)
+from . import _lib as L
+from .core import (Diagonal, Mesh, PCGInfo, SEMOperator, _p, advect, combine, ctx, dot,
+                   dot_acc, empty_field, generateMask, lapl, pcg, to_device)
 
 
 # ------------------------------------------------------------ time stepper
-def _unique(t):
-    out = []
-    for v in t:
-        if not any(v == w for w in out):
-            out.append(v)
-    return out
-
-
 def bdfExtK(t, k: int = 3):
     """BDFk/EXTk coefficients (a, b) for the time history t = [t_{n+1}, t_n,
     ...] (time.jl:31-53): b = first row of derivMat(unique(t)), a =
     interpMat(t_{n+1}, t_past); zero-padded to k (k+1) while the history
-    holds fewer distinct times; steady start (one time) gives a = [1, 0..]."""
-    t = _unique([float(v) for v in t])
-    kk = len(t) - 1
-    a = interpMat(np.array([t[0]]), np.array(t[1:])).reshape(-1) if kk > 0 else np.zeros(0)
-    b = derivMat(np.array(t))[0, :].copy()
-    if kk < k:
-        a = np.concatenate([a, np.zeros(k - kk)])
-        b = np.concatenate([b, np.zeros(k - kk)])
-    else:
-        a, b = a[:k], b[:k + 1]
-    if kk == 0:
-        a[0] = 1.0
+    holds fewer distinct times; steady start (one time) gives a = [1, 0..].
+    Computed by the library's host code (sem_bdfextk) once per step."""
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    a = np.zeros(int(k))
+    b = np.zeros(int(k) + 1)
+    L.call("sem_bdfextk", t.size, t.ctypes.data, int(k), a.ctypes.data, b.ctypes.data)
     return a, b
 
 
@@ -153,6 +142,8 @@ class ConvectionDiffusion:
         self.info = PCGInfo()
         self.iters_total = 0
         self._ts0 = tstep.state()
+        self._cd = None          # the library's time loop (sem_cd), built on first use
+        self._adv_spare: List[torch.Tensor] = []
 
     # -- operators of the current step (cached per bdfB[1])
     def _op(self, b0: float):
@@ -228,9 +219,74 @@ class ConvectionDiffusion:
 
     def step_(self):
         """step!(cdn) (convectionDiffusion.jl:148-155)."""
+        if self.native():
+            self.steps_(1)
+            return
         self.updateHist_()
         self.tstep.updateHist_()
         self.evolve_()
+
+    def native(self) -> bool:
+        """True when the library's time loop (sem_cd_steps) covers this
+        equation: no time-dependent data (default setters, f None, ub zero),
+        the Jacobi preconditioner, the relative tolerance, order <= 4."""
+        return (self.setb is fixU_ and self.setF is fixU_ and self.setnu is None
+                and self.f is None and self.fld._ub_zero and self.precond == "jacobi"
+                and self.abs_tol is None and 1 <= len(self.fld.uh) <= 4)
+
+    def steps_(self, nsteps: int, acc: Optional[torch.Tensor] = None, record=None):
+        """nsteps calls of step! inside the library (sem_cd_steps): the same
+        launches in the same order as step_ issues one by one, with one host
+        wait per step (the PCG stop test). acc: a one-element device tensor
+        receiving += dt/2 sum(B u^2) after each step; record: a list that
+        receives each step's (bdfA, bdfB)."""
+        n = int(nsteps)
+        if n <= 0:
+            return
+        if not self.native():
+            raise ValueError("steps_ needs an equation the library's time loop covers")
+        k = len(self.fld.uh)
+        if self._cd is None:
+            h = C.c_void_p()
+            nu_s, nu_f = ((float(self.nu), None) if isinstance(self.nu, (int, float))
+                          else (0.0, self.nu))
+            L.call("sem_cd_create", ctx(), self.mshV.h, self.mshD.h, int(self.fld.M._sem_bits),
+                   k, nu_s, _p(nu_f), _p(self.vx), _p(self.vy), float(self.tstep.dt),
+                   float(self.tol), int(self.maxiter or 0), C.byref(h))
+            self._cd = h
+            self._cd_fin = weakref.finalize(self, L.load().sem_cd_destroy, h)
+        adv, valid = [], []
+        for e in self.advH:
+            if e is not None:
+                adv.append(e)
+                valid.append(1)
+            else:
+                adv.append(self._adv_spare.pop() if self._adv_spare else self.mshV.zeros())
+                valid.append(0)
+        uh_p = (C.c_void_p * k)(*[t.data_ptr() for t in self.fld.uh])
+        adv_p = (C.c_void_p * k)(*[t.data_ptr() for t in adv])
+        val = (C.c_int * k)(*valid)
+        tt = np.array(self.tstep.time, dtype=np.float64)
+        a_out = np.zeros((n, k)) if record is not None else None
+        b_out = np.zeros((n, k + 1)) if record is not None else None
+        it = C.c_int64()
+        st = L.call("sem_cd_steps", ctx(), self._cd, n, _p(self.fld.u), uh_p, adv_p, val,
+                    tt.ctypes.data, _p(acc), a_out.ctypes.data if a_out is not None else None,
+                    b_out.ctypes.data if b_out is not None else None, C.byref(it))
+        uh_of = {t.data_ptr(): t for t in self.fld.uh}
+        self.fld.uh = [uh_of[uh_p[i]] for i in range(k)]
+        adv_of = {t.data_ptr(): t for t in adv}
+        bufs = [adv_of[adv_p[i]] for i in range(k)]
+        self.advH = [bufs[i] if val[i] else None for i in range(k)]
+        self._adv_spare += [bufs[i] for i in range(k) if not val[i]]
+        self.tstep.time = [float(v) for v in tt]
+        self.tstep.istep += n
+        self.tstep.bdfA, self.tstep.bdfB = bdfExtK(self.tstep.time, self.tstep.k)
+        self.iters_total += it.value
+        if record is not None:
+            record.extend((a_out[s].copy(), b_out[s].copy()) for s in range(n))
+        if st == L.SEM_WARN_MAXITER:
+            L.maxiter_warning(float("nan"))
 
     def simulate_(self, callback: Callable = lambda cdn: None):
         """simulate!(cdn, callback!) (convectionDiffusion.jl:159-179)."""
@@ -245,6 +301,9 @@ class ConvectionDiffusion:
 
     def run_(self, nsteps: int, callback: Optional[Callable] = None):
         """nsteps calls of step!, counted (not by time)."""
+        if callback is None and self.native():
+            self.steps_(nsteps)
+            return
         for _ in range(int(nsteps)):
             self.step_()
             if callback is not None:
@@ -321,12 +380,20 @@ def cd_gradient(cdn: ConvectionDiffusion, u0, nsteps: int,
     snaps = {}
     coefs = {}
     Jd = torch.zeros(1, dtype=torch.float64, device=cdn.fld.u.device)
-    for n in range(1, N + 1):
-        if (n - 1) % c == 0:
-            snaps[n - 1] = cdn.snapshot()
-        cdn.step_()
-        coefs[n] = (cdn.tstep.bdfA.copy(), cdn.tstep.bdfB.copy())
-        cd_objective_step(cdn, Jd)
+    if cdn.native():
+        for m0 in range(0, N, c):
+            snaps[m0] = cdn.snapshot()
+            rec = []
+            cdn.steps_(min(c, N - m0), acc=Jd, record=rec)
+            for s, ab in enumerate(rec):
+                coefs[m0 + 1 + s] = ab
+    else:
+        for n in range(1, N + 1):
+            if (n - 1) % c == 0:
+                snaps[n - 1] = cdn.snapshot()
+            cdn.step_()
+            coefs[n] = (cdn.tstep.bdfA.copy(), cdn.tstep.bdfB.copy())
+            cd_objective_step(cdn, Jd)
     J = float(Jd)
     iters_fwd = cdn.iters_total
     steps = N
@@ -380,6 +447,9 @@ def cd_objective(cdn: ConvectionDiffusion, u0, nsteps: int) -> float:
     """J of ``cd_gradient`` by a plain forward run from u0."""
     cdn.reset_(u0)
     Jd = torch.zeros(1, dtype=torch.float64, device=cdn.fld.u.device)
+    if cdn.native():
+        cdn.steps_(int(nsteps), acc=Jd)
+        return float(Jd)
     for _ in range(int(nsteps)):
         cdn.step_()
         cd_objective_step(cdn, Jd)

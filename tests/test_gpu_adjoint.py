@@ -157,6 +157,40 @@ def test_cd_steps_match_oracle(se):
     assert _rel(se.to_host(cdn.fld.u), ref) < 1e-11
 
 
+def test_cd_library_time_loop_matches_step_by_step(se):
+    """sem_cd_steps (the library's step loop) issues the launches of the
+    step-by-step path in the same order: states, histories, times, the
+    objective and the iteration count are bit-identical, and both match the
+    oracle."""
+    import torch
+    _log("cd library time loop vs step-by-step, 7 steps")
+    cdA, mVo, mDo = _cd_pair(se)
+    cdB, _, _ = _cd_pair(se)
+    assert cdA.native()
+    cdB.native = lambda: False          # the step-by-step host path
+    u0 = np.exp(-((mVo.x - 0.5) ** 2 + mVo.y ** 2) / (2 * 0.3 ** 2))
+    JA = torch.zeros(1, dtype=torch.float64, device="cuda")
+    JB = torch.zeros(1, dtype=torch.float64, device="cuda")
+    cdA.reset_(u0)
+    cdB.reset_(u0)
+    rec = []
+    cdA.steps_(3, acc=JA, record=rec)
+    cdA.steps_(4, acc=JA, record=rec)
+    for n in range(7):
+        cdB.step_()
+        se.cd_objective_step(cdB, JB)
+        np.testing.assert_array_equal(rec[n][0], cdB.tstep.bdfA)
+        np.testing.assert_array_equal(rec[n][1], cdB.tstep.bdfB)
+    np.testing.assert_array_equal(se.to_host(cdA.fld.u), se.to_host(cdB.fld.u))
+    for ha, hb in zip(cdA.fld.uh, cdB.fld.uh):
+        np.testing.assert_array_equal(se.to_host(ha), se.to_host(hb))
+    assert cdA.tstep.time == cdB.tstep.time and cdA.tstep.istep == cdB.tstep.istep == 7
+    assert float(JA) == float(JB) and cdA.iters_total == cdB.iters_total
+    Mo = o.generateMask("DDDD", mVo)
+    ref = o.cd_forward(mVo, mDo, u0, -mVo.y, mVo.x, 1e-2, 1e-2, 7, Mo, tol_rel=1e-14)
+    assert _rel(se.to_host(cdA.fld.u), ref) < 1e-11
+
+
 def test_cd_gradient_matches_oracle_and_checkpointing_is_exact(se):
     _log("cd adjoint 6x6 pts, 3x3 elements, 9 steps")
     cdn, mVo, mDo = _cd_pair(se)
