@@ -176,7 +176,13 @@ int sem_abu(sem_ctx* ctx, int m, int n, const double* u, const double* As, int m
 /* elementwise y = a .* b, a ./ b or alpha ./ a over n entries (y may alias a
  * or b): the products, quotients and inverse of the diagonal operator of the
  * Spectral submodule (DiagonalOp, Spectral/Operators.jl:37-118) */
-enum sem_pointwise_op { SEM_PW_MUL = 0, SEM_PW_DIV = 1, SEM_PW_SCALAR_DIV = 2 };
+enum sem_pointwise_op {
+  SEM_PW_MUL = 0,        /* y = a .* b */
+  SEM_PW_DIV = 1,        /* y = a ./ b */
+  SEM_PW_SCALAR_DIV = 2, /* y = alpha ./ a */
+  SEM_PW_DIV_SCALAR = 3, /* y = a ./ alpha */
+  SEM_PW_SCALE = 4       /* y = alpha .* a */
+};
 int sem_pointwise(sem_ctx* ctx, int op, int64_t n, double alpha, const double* a,
                   const double* b, double* y);
 
@@ -375,7 +381,8 @@ int sem_mesh_interp(sem_ctx* ctx, const sem_mesh* mesh, const double* u, int npt
 /* sum(a .* b .* mult) with the mesh's mult (pcg.jl:45); result to host */
 int sem_wdot(sem_ctx* ctx, const sem_mesh* mesh, const double* a, const double* b,
              double* result_host);
-/* sum(a .* b .* w) over n entries (w NULL: sum(a .* b)); result to host.
+/* sum(a .* b .* w) over n entries (w NULL: sum(a .* b); b NULL: sum(a .* w)
+ * or sum(a)); result to host.
  * The local-node sums of the objectives, e.g. 1/2 sum(B .* e .* e)
  * (semPS.jl:210-211) */
 int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,

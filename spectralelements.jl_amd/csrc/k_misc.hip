@@ -311,7 +311,7 @@ k_dot_part(const double* __restrict__ a, const double* __restrict__ b,
   double acc = 0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long)gridDim.x * blockDim.x)
-    acc += w ? a[t] * b[t] * w[t] : a[t] * b[t];
+    acc += b ? (w ? a[t] * b[t] * w[t] : a[t] * b[t]) : (w ? a[t] * w[t] : a[t]);
   const double s = block_reduce_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -335,7 +335,15 @@ __global__ void k_pointwise(int op, double alpha, const double* a, const double*
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const double av = a[t];
-  y[t] = op == 0 ? av * b[t] : (op == 1 ? av / b[t] : alpha / av);
+  double v;
+  switch (op) {
+    case 0: v = av * b[t]; break;
+    case 1: v = av / b[t]; break;
+    case 2: v = alpha / av; break;
+    case 3: v = av / alpha; break;
+    default: v = alpha * av; break;
+  }
+  y[t] = v;
 }
 
 // out = alpha x + beta y (out may alias x or y)

@@ -730,7 +730,7 @@ int sem_wdot(sem_ctx* ctx, const sem_mesh* m, const double* a, const double* b, 
 
 int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
             double* result) {
-  SEM_ARG(ctx && a && b && result && n >= 0, "bad arguments");
+  SEM_ARG(ctx && a && result && n >= 0, "bad arguments");
   const int np = 256;
   SEM_TRY(ensure_ws(ctx, sizeof(double) * (np + 1)));
   double* part = static_cast<double*>(ctx->ws);
@@ -742,7 +742,7 @@ int sem_dot(sem_ctx* ctx, int64_t n, const double* a, const double* b, const dou
 
 int sem_dot_acc(sem_ctx* ctx, int64_t n, const double* a, const double* b, const double* w,
                 double scale, double* acc) {
-  SEM_ARG(ctx && a && b && acc && n >= 0, "bad arguments");
+  SEM_ARG(ctx && a && acc && n >= 0, "bad arguments");
   const int np = 256;
   SEM_TRY(ensure_ws(ctx, sizeof(double) * np));
   return launch_dot_acc(ctx->stream, n, a, b, w, scale, static_cast<double*>(ctx->ws), np, acc);
@@ -783,8 +783,8 @@ int sem_axpby(sem_ctx* ctx, int64_t n, double alpha, const double* x, double bet
 int sem_pointwise(sem_ctx* ctx, int op, int64_t n, double alpha, const double* a,
                   const double* b, double* y) {
   SEM_ARG(ctx && a && y && n >= 0, "NULL argument");
-  SEM_ARG(op >= SEM_PW_MUL && op <= SEM_PW_SCALAR_DIV, "sem_pointwise: unknown op %d", op);
-  SEM_ARG(op == SEM_PW_SCALAR_DIV || b, "sem_pointwise: op %d needs b", op);
+  SEM_ARG(op >= SEM_PW_MUL && op <= SEM_PW_SCALE, "sem_pointwise: unknown op %d", op);
+  SEM_ARG(op >= SEM_PW_SCALAR_DIV || b, "sem_pointwise: op %d needs b", op);
   return launch_pointwise(ctx->stream, op, n, alpha, a, b, y);
 }
 

@@ -19,7 +19,7 @@ SEM_ERR = {-1: "SEM_ERR_ARG", -2: "SEM_ERR_HIP", -3: "SEM_ERR_UNSUPPORTED",
 SEM_F64, SEM_F32, SEM_F16_MFMA, SEM_BF16_MFMA = 0, 1, 2, 3
 SEM_DEFORM_NONE, SEM_DEFORM_SINSIN, SEM_DEFORM_AFFINE01 = 0, 1, 2
 SEM_PRECOND_IDENTITY, SEM_PRECOND_DIAG, SEM_PRECOND_DIAG_LOCAL = 0, 1, 2
-SEM_PW_MUL, SEM_PW_DIV, SEM_PW_SCALAR_DIV = 0, 1, 2
+SEM_PW_MUL, SEM_PW_DIV, SEM_PW_SCALAR_DIV, SEM_PW_DIV_SCALAR, SEM_PW_SCALE = 0, 1, 2, 3, 4
 FIELD_IDS = {n: i for i, n in enumerate(
     ["x", "y", "Jac", "Jaci", "rx", "ry", "sx", "sy", "B", "Bi", "G11", "G12", "G22", "mult"])}
 
