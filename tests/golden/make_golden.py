@@ -5,7 +5,7 @@ Contents (all float64 unless noted; fields in the local column-major layout):
   gll{p}_z, gll{p}_w, gll{p}_D      GLL nodes, weights, derivMat for p = 8, 9, 10, 16
   c1_l2g (int32), c1_mult, c1_mask  4x4 elements, p = 9, undeformed, bc DDDD
   c1_pcg_x, c1_pcg_iters (int64)    Poisson -Lap u = 2 pi^2 sin sin, Jacobi PCG to
-                                    1e-10 * norm(b, Inf) (the config-1 solve)
+                                    1e-10 * norm(b, Inf) (the 4x4-element N=8 Poisson solve)
   d_x, d_y, d_B, d_G11, d_G12, d_G22, d_Jac
                                     2x2 elements, p = 8, sin-sin map (amplitude 0.1)
   d_u, d_hlmz                       hlmz(u, 1.3, 0.7) on that mesh, u ~ N(0,1) seed 7

@@ -82,35 +82,6 @@ def test_linsolve_gradient_matches_oracle(se):
     assert _rel(se.to_host(g), go) < 1e-9
 
 
-def test_linsolve_gradient_fd_128x128(se):
-    """The steady adjoint workload (128x128 elements, N=7, nu = 1 + 0.5 tanh of
-    16 Fourier modes seed 41, u* from the seed-42 coefficient, f = 1):
-    central differences on 3 random continuous directions (seed 43)."""
-    import torch
-    from semhip import workloads as w
-    _log("linsolve FD 128x128 N=7")
-    m = se.Mesh(8, 8, 128, 128, (False, False), se.deform_sinsin(0.1))
-    M = se.generateMask("DDDD", m)
-    nu = w.coefficient_field(m, 41)
-    f = torch.ones_like(nu)
-    prob = se.VarDiffusion(m, M, f)
-    solver = se.diffusion_solver(tol=1e-13)
-    ustar = se.linsolve(w.coefficient_field(m, 42), prob, solver)
-    J, u, g = _linsolve_grad(se, m, M, nu, f, ustar, 1e-13)
-
-    def Jof(v):
-        uu = se.linsolve(v, prob, solver)
-        e = uu - ustar
-        return 0.5 * float(torch.sum(m.B * e * e))
-    for s in range(3):
-        d = w.continuous(w.global_normal_field(m, 43 + s), m)
-        h = 1e-5
-        fd = (Jof(nu + h * d) - Jof(nu - h * d)) / (2 * h)
-        ad = float(torch.sum(g * d))
-        _log(f"  dir {s}: fd {fd:.12e} adjoint {ad:.12e} rel {abs(fd - ad) / abs(ad):.2e}")
-        assert abs(fd - ad) <= 1e-6 * abs(ad)
-
-
 def _cd_pair(se, n=6, nd=9, E=3, dt=1e-2, nu=1e-2, k=2, precond="jacobi"):
     mV = se.Mesh(n, n, E, E, (False, False), se.deform_sinsin(0.1))
     mD = se.Mesh(nd, nd, E, E, (False, False), se.deform_sinsin(0.1))

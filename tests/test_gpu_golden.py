@@ -1,6 +1,6 @@
 """The library on the GPU against the committed golden fixtures: numbering,
 multiplicity and mask bit-exact; deformed geometry, one Helmholtz apply within
-1e-12; one periodic gather-scatter value-exact; the config-1 Jacobi-PCG
+1e-12; one periodic gather-scatter value-exact; the 4x4-element N=8 Poisson Jacobi-PCG
 solution within the solve tolerance and its iteration count within 1%."""
 import os
 
@@ -28,7 +28,7 @@ def _rel(a, b):
     return np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def test_config1_structure_and_solve(se, gold):
+def test_poisson_4x4_N8_structure_and_solve(se, gold):
     m = se.Mesh(9, 9, 4, 4, (False, False), se.fixU)
     M = se.generateMask("DDDD", m)
     np.testing.assert_array_equal(se.to_host(m.l2g.double()).astype(np.int32), gold["c1_l2g"])

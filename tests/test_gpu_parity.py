@@ -187,10 +187,10 @@ def test_fused_apply_is_deterministic(se):
         assert torch.equal(op(u), a)
 
 
-def test_fused_apply_full_config2_size(se):
+def test_fused_apply_256x256_N7_full_size(se):
     """256x256 elements, N=7 (4.19 M local nodes), deformed, Helmholtz: the
     headline operator against the oracle at full size."""
-    _log("apply cfg2 size")
+    _log("apply 256x256 N=7")
     m, mo = _pair(se, 8, 256, (False, False), "sinsin")
     M = se.generateMask("DDDD", m)
     Mo = o.generateMask("DDDD", mo)
@@ -211,7 +211,7 @@ def test_jacobi_matches_oracle(se):
     assert _rel(d, ref) < 1e-13
 
 
-def _cfg1(se):
+def _poisson_4x4_n9(se):
     m, mo = _pair(se, 9, 4, (False, False), "none")
     M = se.generateMask("DDDD", m)
     Mo = o.generateMask("DDDD", mo)
@@ -220,9 +220,9 @@ def _cfg1(se):
     return m, mo, M, Mo, b
 
 
-def test_pcg_config1_matches_oracle_and_exact_solution(se):
-    _log("pcg cfg1")
-    m, mo, M, Mo, b = _cfg1(se)
+def test_pcg_poisson_4x4_N8_matches_oracle_and_exact_solution(se):
+    _log("pcg Poisson 4x4 N=8")
+    m, mo, M, Mo, b = _poisson_4x4_n9(se)
     tol = 1e-10 * np.abs(b).max()
     info_o = o.PCGInfo()
     uo = o.pcg(b, lambda v: o.opLHS(v, 1.0, 0.0, mo, Mo), mult=mo.mult, tol=tol, info=info_o)
@@ -303,7 +303,7 @@ def test_pcg_relative_tolerance_on_device_and_dot_acc(se):
 
 def test_pcg_maxiter_and_zero_rhs(se):
     _log("pcg maxiter / zero rhs")
-    m, mo, M, Mo, b = _cfg1(se)
+    m, mo, M, Mo, b = _poisson_4x4_n9(se)
     op = se.SEMOperator(1.0, 0.0, m, M)
     info = se.PCGInfo()
     with pytest.warns(RuntimeWarning, match="warning: res"):

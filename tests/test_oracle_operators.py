@@ -99,7 +99,7 @@ def test_jacobi_diag_matches_dense_assembled_diagonal():
     assert np.all(dinv[M == 0] == 0)
 
 
-def _cfg1_solve(n, Ex, tol_rel=1e-12):
+def _poisson_solve(n, Ex, tol_rel=1e-12):
     m = o.Mesh(n, n, Ex, Ex, dense_gs=False)
     M = o.generateMask("DDDD", m)
     f = 2 * np.pi ** 2 * np.sin(np.pi * m.x) * np.sin(np.pi * m.y)
@@ -112,11 +112,11 @@ def _cfg1_solve(n, Ex, tol_rel=1e-12):
 
 def test_manufactured_solution_converges_spectrally():
     errs = []
-    for n in (5, 7, 9, 11):  # points per direction; config 1 is n = 9 (N = 8)
-        m, u, info = _cfg1_solve(n, 4)
+    for n in (5, 7, 9, 11):  # points per direction; the 4x4-element case is n = 9 (N = 8)
+        m, u, info = _poisson_solve(n, 4)
         assert info.converged
         errs.append(np.abs(u - np.sin(np.pi * m.x) * np.sin(np.pi * m.y)).max())
-    assert errs[2] < 2e-10    # config 1 (N=8): spectral accuracy
+    assert errs[2] < 2e-10    # N = 8: spectral accuracy
     for i in range(len(errs) - 1):
         assert errs[i + 1] < errs[i] / 20
 
@@ -154,7 +154,7 @@ def test_pcg_matches_direct_global_solve():
 
 
 def test_pcg_maxiter_warns_and_returns():
-    m, _, _ = _cfg1_solve(5, 2)
+    m, _, _ = _poisson_solve(5, 2)
     M = o.generateMask("DDDD", m)
     b = m.gs(M * o.mass(np.ones(m.x.shape), m))
     info = o.PCGInfo()
