@@ -335,9 +335,87 @@ def cpu_baseline(budget_s=15.0):
         th = max(i.get("num_threads", 1) for i in threadpoolctl.threadpool_info())
     except Exception:
         th = os.cpu_count()
+    # Jacobi-PCG on the CPU: the 4x4-element N=8 Poisson solve to 1e-10
+    # relative, whole; the 256x256 N=7 Helmholtz solve timed over 3
+    # iterations and scaled to the GPU solve's iteration count
+    p4 = o.Mesh(9, 9, 4, 4, (False, False), o.fixU, dense_gs=True)
+    M4 = o.generateMask("DDDD", p4)
+    f4 = 2 * np.pi ** 2 * np.sin(np.pi * p4.x) * np.sin(np.pi * p4.y)
+    b4 = p4.gs(M4 * o.mass(f4, p4))
+    info4 = o.PCGInfo()
+    t0 = time.perf_counter()
+    o.pcg(b4, lambda v: o.opLHS(v, 1.0, 0.0, p4, M4), mult=p4.mult,
+          tol=1e-10 * np.abs(b4).max(), info=info4)
+    t_small = time.perf_counter() - t0
+    b = mo.gs(Mo * o.mass(np.random.default_rng(3).standard_normal(mo.x.shape), mo))
+    dinv = o.jacobi_diag(mo, 1.0, 1.0, Mo)
+    t0 = time.perf_counter()
+    o.pcg(b, lambda v: o.opLHS(v, 1.0, 1.0, mo, Mo), opM=lambda r: dinv * r, mult=mo.mult,
+          tol=0.0, maxiter=3)
+    t_it = (time.perf_counter() - t0) / 3
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")),
+                         "")
+    except OSError:
+        pass
     return {"value": mo.nglobal / dt / 1e9, "unit": "GDOF/s", "cores": th, "kind": "port",
+            "cpu_model": model,
             "sample": f"{reps} applies of the reference-faithful NumPy/OpenBLAS apply "
-                      f"(dense QQ' GEMMs) at 256x256 N=7, {dt * 1e3:.1f} ms each"}
+                      f"(dense QQ' GEMMs) at 256x256 N=7, {dt * 1e3:.1f} ms each",
+            "pcg_poisson_4x4_N8_s": t_small, "pcg_poisson_4x4_N8_iters": info4.iters,
+            "pcg_helmholtz_256x256_N7_s_per_iter": t_it}
+
+
+def probe(se, kind):
+    """Short fixed workloads for counter collection (rocprofv3 --pmc):
+    'mfma': 11 assembled applies of each 16x16-block operator form (FP64,
+    FP32, FP16, BF16, FP16 x3, BF16 x3) on 128x128 elements, N=15;
+    'pcg': 100 Jacobi-PCG iterations at 256x256, N=7 (FP64) and at 128x128,
+    N=15 (FP32); 'cd': 60 cd2d steps (64x64, N=9) in the library's loop."""
+    import warnings
+    import torch
+    from semhip import workloads as W
+    warnings.simplefilter("ignore")
+    if kind == "mfma":
+        m = se.Mesh(16, 16, 128, 128, (False, False), se.deform_sinsin(0.1))
+        M = se.generateMask("DDDD", m)
+        u = W.global_normal_field(m, 2)
+        for prec in ["f64", "f32", "f16", "bf16", "f16x3", "bf16x3"]:
+            op = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
+            uu = u if prec == "f64" else u.float()
+            y = op(uu)
+            for _ in range(10):
+                op(uu, out=y)
+            torch.cuda.synchronize()
+            log(f"probe mfma {prec} done")
+    elif kind == "pcg":
+        m = se.Mesh(8, 8, 256, 256, (False, False), se.deform_sinsin(0.1))
+        M = se.generateMask("DDDD", m)
+        op = se.SEMOperator(1.0, 1.0, m, M)
+        b = W.helmholtz_rhs(se, m, M, seed=3)
+        se.pcg(b, op, opM=se.Diagonal(op.jacobi()), mult=m.mult, tol=0.0, maxiter=100)
+        torch.cuda.synchronize()
+        log("probe pcg fp64 done")
+        m2 = se.Mesh(16, 16, 128, 128, (False, False), se.deform_sinsin(0.1))
+        M2 = se.generateMask("DDDD", m2)
+        op32 = se.SEMOperator(1.0, 1.0, m2, M2, precision="f32")
+        op64 = se.SEMOperator(1.0, 1.0, m2, M2)
+        b2 = W.helmholtz_rhs(se, m2, M2, seed=3).float()
+        se.pcg(b2, op32, opM=se.Diagonal(op64.jacobi().float()), mult=m2.mult, tol=0.0,
+               maxiter=100)
+        torch.cuda.synchronize()
+        log("probe pcg fp32 done")
+    elif kind == "cd":
+        mV = se.Mesh(10, 10, 64, 64, (False, False), se.deform_sinsin(0.1))
+        mD = se.Mesh(15, 15, 64, 64, (False, False), se.deform_sinsin(0.1))
+        fld = se.Field("DDDD", mV, 2)
+        cdn = se.ConvectionDiffusion("cd2d", fld, -mV.y, mV.x,
+                                     se.TimeStepper(0.0, 60e-3, 1e-3, 2), mD, nu=1e-3, tol=1e-13)
+        se.cd_objective(cdn, W.gaussian_blob(mV), 60)
+        torch.cuda.synchronize()
+        log("probe cd done")
 
 
 def main():
@@ -352,6 +430,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cd", action="store_true")
     ap.add_argument("--cd-steps", type=int, default=2000)
+    ap.add_argument("--probe", choices=["mfma", "pcg", "cd"],
+                    help="run only a short counter-collection workload and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -363,6 +443,9 @@ def main():
     import torch
     import semhip as se
     torch.cuda.set_device(0)
+    if args.probe:
+        probe(se, args.probe)
+        return
     log("apply")
     m, ms, ms_one, sets = bench_apply(se, args)
     nl, ng = m.nlocal, m.nglobal
@@ -381,6 +464,8 @@ def main():
         "ms_per_step_single_set": ms_one,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/apply_256x256_N7_hbm_traffic.md (rocprofv3 "
+                                       "--pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
                      "kernel": "fused apply (k_ax strip kernel + k_seams), 48 B per local node"},
     }
     if args.calibrate_copy:
@@ -408,6 +493,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline()
+        cb = res["cpu_baseline"]
+        if "pcg_fp64" in res:
+            cb["pcg_helmholtz_256x256_N7_extrapolated_s"] = (
+                cb["pcg_helmholtz_256x256_N7_s_per_iter"] * res["pcg_fp64"]["iters"])
     print(json.dumps(res), flush=True)
 
 
