@@ -120,6 +120,28 @@ def bench_pcg_mg(se, m, op, b, tol, x_jacobi, t_jacobi, reps=3):
         walls.append(time.perf_counter() - t0)
     wall = sorted(walls)[len(walls) // 2]
     bmax = float(b.abs().max())
+    # the V-cycle in FP32 (the PCG in FP64, flexible beta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P32 = se.MG(op, precision="f32")
+    torch.cuda.synchronize()
+    t_setup32 = time.perf_counter() - t0
+    walls32 = []
+    for _ in range(reps):
+        info32 = se.PCGInfo()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x32 = se.pcg(b, op, opM=P32, mult=m.mult, tol=tol, info=info32)
+        torch.cuda.synchronize()
+        walls32.append(time.perf_counter() - t0)
+    wall32 = sorted(walls32)[len(walls32) // 2]
+    fp32_cycle = {"iters": info32.iters, "setup_s": t_setup32, "solve_s": wall32,
+                  "time_to_solution_s": t_setup32 + wall32,
+                  "speedup_vs_fp64_cycle_solve": wall / wall32,
+                  "rel_res_true": float((b - op(x32)).abs().max()) / bmax,
+                  "converged": info32.converged,
+                  "rel_diff_vs_fp64_cycle": float((x32 - x).abs().max()) / float(x.abs().max())}
+    del P32, x32
     return {"levels": [list(lv[:3]) for lv in P.levels()], "iters": info.iters,
             "setup_s": t_setup, "solve_s": wall, "device_ms": info.elapsed_ms,
             "ms_per_iter": info.elapsed_ms / max(info.iters, 1),
@@ -129,7 +151,8 @@ def bench_pcg_mg(se, m, op, b, tol, x_jacobi, t_jacobi, reps=3):
             "rel_res_true": float((b - op(x)).abs().max()) / bmax,
             "converged": info.converged,
             "rel_diff_vs_jacobi": float((x - x_jacobi).abs().max()) / float(x_jacobi.abs().max()),
-            "smoother": "Chebyshev-Jacobi degree 3, range 20, coarsest degree 8"}
+            "smoother": "Chebyshev-Jacobi degree 3, range 20, coarsest degree 8",
+            "fp32_cycle": fp32_cycle}
 
 
 # (contraction variant, inner_tol): every variant runs at the configuration's
@@ -169,6 +192,39 @@ def bench_mixed(se):
                    "us_per_iter": info.elapsed_ms / max(info.iters, 1) * 1e3,
                    "rel_res_true": float((b - op64(x64)).abs().max()) / bmax,
                    "rel_err_vs_ref": float((x64 - xref).abs().max()) / xrmax}
+    # FP64 with the multigrid V-cycle as opM, same tolerance (setup timed apart)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mg = se.MG(op64)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    info = se.PCGInfo()
+    t0 = time.perf_counter()
+    xmg = se.pcg(b, op64, opM=mg, mult=m.mult, tol=tol, info=info)
+    torch.cuda.synchronize()
+    t_mg = time.perf_counter() - t0
+    out["fp64_mg"] = {"iters": info.iters, "setup_s": t_setup, "time_s": t_mg,
+                      "time_to_solution_s": t_setup + t_mg, "speedup_vs_fp64": t64 / t_mg,
+                      "rel_res_true": float((b - op64(xmg)).abs().max()) / bmax,
+                      "rel_err_vs_ref": float((xmg - xref).abs().max()) / xrmax}
+    del mg, xmg
+    # mixed: the same PCG in FP64 with the V-cycle in FP32
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mg = se.MG(op64, precision="f32")
+    torch.cuda.synchronize()
+    t_setup32 = time.perf_counter() - t0
+    info = se.PCGInfo()
+    t0 = time.perf_counter()
+    xmg = se.pcg(b, op64, opM=mg, mult=m.mult, tol=tol, info=info)
+    torch.cuda.synchronize()
+    t_mg32 = time.perf_counter() - t0
+    out["mg_fp32_cycle"] = {"iters": info.iters, "setup_s": t_setup32, "time_s": t_mg32,
+                            "time_to_solution_s": t_setup32 + t_mg32,
+                            "speedup_vs_fp64": t64 / t_mg32, "speedup_vs_fp64_mg": t_mg / t_mg32,
+                            "rel_res_true": float((b - op64(xmg)).abs().max()) / bmax,
+                            "rel_err_vs_ref": float((xmg - xref).abs().max()) / xrmax}
+    del mg, xmg
     for prec, delta in MIXED_VARIANTS:
         op_lo = se.SEMOperator(1.0, 1.0, m, M, precision=prec)
         for key, dl in (("", delta), ("_inner_tol_1e-5", 1e-5)):

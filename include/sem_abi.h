@@ -318,6 +318,10 @@ typedef struct sem_mg_params {
   int npow;           /* default 20 */
   double safety;      /* default 1.1 */
   int max_levels;     /* default 32 */
+  int precision;      /* SEM_F64 (default) or SEM_F32: the V-cycle (smoothing,
+                         transfers, level operators) in FP32, with the FP64
+                         eigenvalue bounds; sem_pcg_mg then uses the flexible
+                         (Polak-Ribiere) beta, the rest of the solve in FP64 */
 } sem_mg_params;
 
 typedef struct sem_mg sem_mg;

@@ -24,39 +24,45 @@ namespace {
 
 inline int nblk(long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
+// The V-cycle's vector kernels are typed by the cycle's precision T (FP64,
+// or FP32 for the FP32 cycle); the coefficients come in FP64 from the host.
+
 // Chebyshev start: d = dinv .* r / theta; x = d (x0 = 0, and r = b copied
 // from b) or x += d.
-__global__ void k_cheb_start(const double* __restrict__ b, double* __restrict__ r,
-                             const double* __restrict__ dinv, double* __restrict__ d,
-                             double* __restrict__ x, double theta, int from_zero, long n) {
+template <typename T>
+__global__ void k_cheb_start(const T* __restrict__ b, T* __restrict__ r,
+                             const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ x,
+                             double theta, int from_zero, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  double rv;
+  T rv;
   if (from_zero) {
     rv = b[t];
     r[t] = rv;
   } else {
     rv = r[t];
   }
-  const double dv = dinv[t] * rv / theta;
+  const T dv = dinv[t] * rv / T(theta);
   d[t] = dv;
   x[t] = from_zero ? dv : x[t] + dv;
 }
 
 // One Chebyshev step after w = A d: r -= w; d = c1 d + c2 dinv .* r; x += d
-__global__ void k_cheb_step(const double* __restrict__ w, double* __restrict__ r,
-                            const double* __restrict__ dinv, double* __restrict__ d,
-                            double* __restrict__ x, double c1, double c2, long n) {
+template <typename T>
+__global__ void k_cheb_step(const T* __restrict__ w, T* __restrict__ r,
+                            const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ x,
+                            double c1, double c2, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double rv = r[t] - w[t];
+  const T rv = r[t] - w[t];
   r[t] = rv;
-  const double dv = c1 * d[t] + c2 * (dinv[t] * rv);
+  const T dv = T(c1) * d[t] + T(c2) * (dinv[t] * rv);
   d[t] = dv;
   x[t] = x[t] + dv;
 }
 
-__global__ void k_sub(double* __restrict__ r, const double* __restrict__ w, long n) {
+template <typename T>
+__global__ void k_sub(T* __restrict__ r, const T* __restrict__ w, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) r[t] = r[t] - w[t];
 }
@@ -70,8 +76,9 @@ struct Xfer {
 };
 
 // e = P xc (per fine element: Jx xc Jy'), x += e
-__global__ void k_prolong(const double* __restrict__ xc, const double* __restrict__ J,
-                          double* __restrict__ e, double* __restrict__ x, Xfer g, long n) {
+template <typename T>
+__global__ void k_prolong(const T* __restrict__ xc, const T* __restrict__ J, T* __restrict__ e,
+                          T* __restrict__ x, Xfer g, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = (int)(t % g.Lf), j = (int)(t / g.Lf);
@@ -80,13 +87,13 @@ __global__ void k_prolong(const double* __restrict__ xc, const double* __restric
   const int sx = exf & g.hx, sy = eyf & g.hy;
   const int exc = exf >> g.hx, eyc = eyf >> g.hy;
   const int blk = g.pf * g.pc;
-  const double* jx = J + sx * blk + a * g.pc;
-  const double* jy = J + (g.hx ? 2 : 1) * blk + sy * blk + b * g.pc;
+  const T* jx = J + sx * blk + a * g.pc;
+  const T* jy = J + (g.hx ? 2 : 1) * blk + sy * blk + b * g.pc;
   const long c0 = (long)exc * g.pc + (long)g.Lc * ((long)eyc * g.pc);
-  double v = 0.0;
+  T v = T(0);
   for (int bb = 0; bb < g.pc; ++bb) {
-    const double wy = jy[bb];
-    double s = 0.0;
+    const T wy = jy[bb];
+    T s = T(0);
     for (int aa = 0; aa < g.pc; ++aa) s += jx[aa] * xc[c0 + aa + (long)g.Lc * bb];
     v += wy * s;
   }
@@ -96,8 +103,9 @@ __global__ void k_prolong(const double* __restrict__ xc, const double* __restric
 
 // rc (element-local) = Jx' (mult_f .* rf) Jy summed over the fine nodes of
 // the coarse element; assembled and masked afterwards by the coarse QQ'
-__global__ void k_restrict(const double* __restrict__ rf, const double* __restrict__ J,
-                           double* __restrict__ rc, Xfer g, long n) {
+template <typename T>
+__global__ void k_restrict(const T* __restrict__ rf, const T* __restrict__ J, T* __restrict__ rc,
+                           Xfer g, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = (int)(t % g.Lc), j = (int)(t / g.Lc);
@@ -105,19 +113,19 @@ __global__ void k_restrict(const double* __restrict__ rf, const double* __restri
   const int eyc = j / g.pc, b = j - eyc * g.pc;
   const int blk = g.pf * g.pc;
   const int nsx = g.hx ? 2 : 1, nsy = g.hy ? 2 : 1;
-  const double* Jy0 = J + nsx * blk;
-  double v = 0.0;
+  const T* Jy0 = J + nsx * blk;
+  T v = T(0);
   for (int sy = 0; sy < nsy; ++sy) {
     const int eyf = (eyc << g.hy) + sy;
     for (int bf = 0; bf < g.pf; ++bf) {
       const int jf = eyf * g.pf + bf;
-      const double wy = Jy0[sy * blk + bf * g.pc + b] * mult1(jf, g.pf, g.NCf, g.per_y);
-      double s = 0.0;
+      const T wy = Jy0[sy * blk + bf * g.pc + b] * T(mult1(jf, g.pf, g.NCf, g.per_y));
+      T s = T(0);
       for (int sx = 0; sx < nsx; ++sx) {
         const int exf = (exc << g.hx) + sx;
         for (int af = 0; af < g.pf; ++af) {
           const int ifn = exf * g.pf + af;
-          s += J[sx * blk + af * g.pc + a] * mult1(ifn, g.pf, g.Lf, g.per_x) *
+          s += J[sx * blk + af * g.pc + a] * T(mult1(ifn, g.pf, g.Lf, g.per_x)) *
                rf[ifn + (long)g.Lf * jf];
         }
       }
@@ -225,14 +233,22 @@ struct sem_mg {
     int h = 0;            // the next level merges this level's elements 2 x 2
     double* J = nullptr;  // transfer rows from the next level onto this one
     Xfer xf{};
+    long nJ = 0;
     double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *w = nullptr, *e = nullptr;
+    // the FP32 cycle: the level operator in FP32 and FP32 copies of the above
+    const sem_operator* op32 = nullptr;
+    float *fx = nullptr, *fb = nullptr, *fr = nullptr, *fd = nullptr, *fw = nullptr,
+          *fe = nullptr, *fdinv = nullptr, *fJ = nullptr;
   };
   sem_mg_params prm{};
   int device = 0;
   const sem_operator* op = nullptr;
+  bool f32 = false;
   std::vector<Level> lv;
-  // PCG vectors on the fine level, reduction partials and the scalar slot
-  double *pr = nullptr, *pp = nullptr, *pw = nullptr, *pz = nullptr, *part = nullptr;
+  // PCG vectors on the fine level (pq: the previous residual of the flexible
+  // recurrence), reduction partials and the scalar slot
+  double *pr = nullptr, *pp = nullptr, *pw = nullptr, *pz = nullptr, *pq = nullptr,
+         *part = nullptr;
 };
 
 namespace {
@@ -250,18 +266,40 @@ void mg_free(sem_mg* mg) {
       if (L.x) (void)hipFree(L.x);
       if (L.b) (void)hipFree(L.b);
     }
+    float* fbufs[] = {L.fx, L.fb, L.fr, L.fd, L.fw, L.fe, L.fdinv, L.fJ};
+    for (float* p : fbufs)
+      if (p) (void)hipFree(p);
+    if (L.op32) sem_operator_destroy(const_cast<sem_operator*>(L.op32));
     if (L.own_op) sem_operator_destroy(const_cast<sem_operator*>(L.op));
     if (L.own_mesh) sem_mesh_destroy(L.mesh);
   }
-  double* bufs[] = {mg->pr, mg->pp, mg->pw, mg->pz, mg->part};
+  double* bufs[] = {mg->pr, mg->pp, mg->pw, mg->pz, mg->pq, mg->part};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   delete mg;
 }
 
-int dalloc(double** p, long n) {
-  SEM_HIP(hipMalloc(p, sizeof(double) * std::max<long>(n, 1)));
+template <typename T>
+int dalloc(T** p, long n) {
+  SEM_HIP(hipMalloc(p, sizeof(T) * std::max<long>(n, 1)));
   return SEM_OK;
+}
+
+// a level's buffers and operator in the cycle's precision
+template <typename T>
+struct LvBuf {
+  T *x, *b, *r, *d, *w, *e, *dinv, *J;
+  const sem_operator* op;
+};
+template <typename T>
+LvBuf<T> bufs(sem_mg::Level& L);
+template <>
+LvBuf<double> bufs<double>(sem_mg::Level& L) {
+  return {L.x, L.b, L.r, L.d, L.w, L.e, L.dinv, L.J, L.op};
+}
+template <>
+LvBuf<float> bufs<float>(sem_mg::Level& L) {
+  return {L.fx, L.fb, L.fr, L.fd, L.fw, L.fe, L.fdinv, L.fJ, L.op32};
 }
 
 // host value of a device reduction result
@@ -285,60 +323,68 @@ int norm_host(sem_ctx* ctx, sem_mg* mg, long n, const double* a, double* out) {
   return fetch(ctx, mg->part + kParts, out);
 }
 
-int apply(sem_ctx* ctx, const sem_operator* op, const double* u, double* y) {
+int apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y) {
   return sem_operator_apply(ctx, op, u, y);
 }
 
 // Chebyshev iteration of degree deg for D^-1 A on [lmax / ratio, lmax]; from
 // zero (x = 0, r = b) or from (x, r). With want_r the residual of the final
 // x is formed in r (one more apply).
-int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& L, const double* b, int from_zero, int deg,
+template <typename T>
+int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& Lv, const T* b, int from_zero, int deg,
           bool want_r) {
   hipStream_t st = ctx->stream;
-  const long n = (long)L.mesh->nl;
-  const double lmax = L.lmax, lmin = L.lmax / mg->prm.ratio;
+  const LvBuf<T> L = bufs<T>(Lv);
+  const long n = (long)Lv.mesh->nl;
+  const double lmax = Lv.lmax, lmin = Lv.lmax / mg->prm.ratio;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
-  hipLaunchKernelGGL(k_cheb_start, dim3(nblk(n)), dim3(256), 0, st, b, L.r, L.dinv, L.d, L.x,
-                     theta, from_zero, n);
+  hipLaunchKernelGGL((k_cheb_start<T>), dim3(nblk(n)), dim3(256), 0, st, b, L.r, L.dinv, L.d,
+                     L.x, theta, from_zero, n);
   SEM_HIP(hipGetLastError());
   for (int i = 1; i < deg; ++i) {
     SEM_TRY(apply(ctx, L.op, L.d, L.w));
     const double rho_n = 1.0 / (2.0 * sigma - rho);
-    hipLaunchKernelGGL(k_cheb_step, dim3(nblk(n)), dim3(256), 0, st, L.w, L.r, L.dinv, L.d, L.x,
-                       rho_n * rho, 2.0 * rho_n / delta, n);
+    hipLaunchKernelGGL((k_cheb_step<T>), dim3(nblk(n)), dim3(256), 0, st, L.w, L.r, L.dinv, L.d,
+                       L.x, rho_n * rho, 2.0 * rho_n / delta, n);
     SEM_HIP(hipGetLastError());
     rho = rho_n;
   }
   if (want_r) {
     SEM_TRY(apply(ctx, L.op, L.d, L.w));
-    hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
+    hipLaunchKernelGGL((k_sub<T>), dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
     SEM_HIP(hipGetLastError());
   }
   return SEM_OK;
 }
 
-// V-cycle on level l: L.x = V(L.b)
+// V-cycle on level l: x = V(b) in the level buffers of precision T
+template <typename T>
 int vcycle(sem_ctx* ctx, sem_mg* mg, int l) {
   hipStream_t st = ctx->stream;
-  sem_mg::Level& L = mg->lv[l];
+  sem_mg::Level& Lv = mg->lv[l];
+  const LvBuf<T> L = bufs<T>(Lv);
   const int last = (int)mg->lv.size() - 1;
-  if (l == last) return cheby(ctx, mg, L, L.b, 1, mg->prm.degree_coarse, false);
-  SEM_TRY(cheby(ctx, mg, L, L.b, 1, mg->prm.degree, true));
-  sem_mg::Level& C = mg->lv[l + 1];
-  const long nc = (long)C.mesh->nl;
-  hipLaunchKernelGGL(k_restrict, dim3(nblk(nc)), dim3(256), 0, st, L.r, L.J, C.b, L.xf, nc);
+  if (l == last) return cheby<T>(ctx, mg, Lv, L.b, 1, mg->prm.degree_coarse, false);
+  SEM_TRY(cheby<T>(ctx, mg, Lv, L.b, 1, mg->prm.degree, true));
+  sem_mg::Level& Cv = mg->lv[l + 1];
+  const LvBuf<T> C = bufs<T>(Cv);
+  const long nc = (long)Cv.mesh->nl;
+  hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(nc)), dim3(256), 0, st, L.r, L.J, C.b, Lv.xf,
+                     nc);
   SEM_HIP(hipGetLastError());
-  SEM_TRY(launch_gsmap_masked(st, C.mesh->map, C.mesh->layout(), C.op->desc.bc_bits, C.b, C.b));
-  SEM_TRY(vcycle(ctx, mg, l + 1));
-  const long n = (long)L.mesh->nl;
-  hipLaunchKernelGGL(k_prolong, dim3(nblk(n)), dim3(256), 0, st, C.x, L.J, L.e, L.x, L.xf, n);
+  SEM_TRY(launch_gsmap_masked(st, Cv.mesh->map, Cv.mesh->layout(), Cv.op->desc.bc_bits, C.b,
+                              C.b));
+  SEM_TRY(vcycle<T>(ctx, mg, l + 1));
+  const long n = (long)Lv.mesh->nl;
+  hipLaunchKernelGGL((k_prolong<T>), dim3(nblk(n)), dim3(256), 0, st, C.x, L.J, L.e, L.x, Lv.xf,
+                     n);
   SEM_HIP(hipGetLastError());
   SEM_TRY(apply(ctx, L.op, L.e, L.w));
-  hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
+  hipLaunchKernelGGL((k_sub<T>), dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
   SEM_HIP(hipGetLastError());
-  return cheby(ctx, mg, L, L.b, 0, mg->prm.degree, false);
+  return cheby<T>(ctx, mg, Lv, L.b, 0, mg->prm.degree, false);
 }
 
 // lmax = safety x Rayleigh quotient after npow power steps of D^-1 A
@@ -446,6 +492,7 @@ int build(sem_ctx* ctx, sem_mg* mg) {
         std::vector<double> J = interp_rows(pf, xo.data(), pc, zc.data());
         Jall.insert(Jall.end(), J.begin(), J.end());
       }
+    Fl.nJ = (long)Jall.size();
     SEM_TRY(dalloc(&Fl.J, (long)Jall.size()));
     SEM_HIP(hipMemcpy(Fl.J, Jall.data(), sizeof(double) * Jall.size(), hipMemcpyHostToDevice));
     // coarse coordinates: the fine geometry at the coarse nodes
@@ -489,11 +536,36 @@ int build(sem_ctx* ctx, sem_mg* mg) {
     SEM_TRY(sem_operator_jacobi(ctx, L.op, L.dinv));
     SEM_TRY(estimate_lmax(ctx, mg, L));   // uses the level's d, w, e only
   }
+  // the FP32 cycle: every level's operator rebuilt in FP32 and FP32 copies
+  // of the diagonals and transfer rows (the eigenvalue bounds are the FP64
+  // ones)
+  if (mg->f32) {
+    for (size_t l = 0; l < mg->lv.size(); ++l) {
+      sem_mg::Level& L = mg->lv[l];
+      const long n = (long)L.mesh->nl;
+      sem_op_desc d{};
+      d.nu = op0->desc.nu;
+      d.k = op0->desc.k;
+      d.bc_bits = op0->desc.bc_bits;
+      d.assemble = 1;
+      sem_operator* o32 = nullptr;
+      SEM_TRY(sem_operator_create(ctx, L.mesh, &d, SEM_F32, &o32));
+      L.op32 = o32;
+      float** fb[] = {&L.fx, &L.fb, &L.fr, &L.fd, &L.fw, &L.fe, &L.fdinv};
+      for (float** q : fb) SEM_TRY(dalloc(q, n));
+      SEM_TRY(launch_convert_f64_f32(st, n, L.dinv, L.fdinv));
+      if (L.nJ > 0) {
+        SEM_TRY(dalloc(&L.fJ, L.nJ));
+        SEM_TRY(launch_convert_f64_f32(st, L.nJ, L.J, L.fJ));
+      }
+    }
+  }
   const long n0 = (long)op0->mesh->nl;
   SEM_TRY(dalloc(&mg->pr, n0));
   SEM_TRY(dalloc(&mg->pp, n0));
   SEM_TRY(dalloc(&mg->pw, n0));
   SEM_TRY(dalloc(&mg->pz, n0));
+  if (mg->f32) SEM_TRY(dalloc(&mg->pq, n0));
   SEM_HIP(hipStreamSynchronize(st));
   return SEM_OK;
 }
@@ -525,6 +597,7 @@ int sem_mg_default_params(sem_mg_params* p) {
   p->npow = 20;
   p->safety = 1.1;
   p->max_levels = 32;
+  p->precision = SEM_F64;
   return SEM_OK;
 }
 
@@ -538,11 +611,14 @@ int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* par
   SEM_ARG(p.degree >= 1 && p.degree_coarse >= 1 && p.ratio > 1.0 && p.npow >= 1 &&
               p.safety >= 1.0 && p.max_levels >= 1,
           "sem_mg_create: bad parameters");
+  SEM_ARG(p.precision == SEM_F64 || p.precision == SEM_F32,
+          "sem_mg_create: precision must be SEM_F64 or SEM_F32");
   sem_mg* mg = new (std::nothrow) sem_mg();
   SEM_ARG(mg, "out of host memory");
   mg->prm = p;
   mg->device = ctx->device;
   mg->op = op;
+  mg->f32 = p.precision == SEM_F32;
   if (dalloc(&mg->part, kParts + 1) != SEM_OK) {
     mg_free(mg);
     return SEM_ERR_HIP;
@@ -590,9 +666,15 @@ int sem_mg_vcycle(sem_ctx* ctx, sem_mg* mg, const double* r, double* z) {
   SEM_ARG(ctx && mg && r && z, "NULL argument");
   SEM_ARG(r != z, "sem_mg_vcycle: r and z must not alias");
   sem_mg::Level& L0 = mg->lv[0];
+  if (mg->f32) {
+    const long n = (long)L0.mesh->nl;
+    SEM_TRY(launch_convert_f64_f32(ctx->stream, n, r, L0.fb));
+    SEM_TRY(vcycle<float>(ctx, mg, 0));
+    return launch_convert_f32_f64(ctx->stream, n, L0.fx, z);
+  }
   L0.b = const_cast<double*>(r);
   L0.x = z;
-  const int s = vcycle(ctx, mg, 0);
+  const int s = vcycle<double>(ctx, mg, 0);
   L0.b = nullptr;
   L0.x = nullptr;
   return s;
@@ -626,8 +708,18 @@ int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b
     if (k == 1) {
       SEM_HIP(hipMemcpyAsync(mg->pp, mg->pz, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     } else {
-      SEM_TRY(launch_axpby(st, n, 1.0, mg->pz, t / t_prev, mg->pp, mg->pp));
+      // FP32 cycle: the flexible (Polak-Ribiere) beta = <z_k, r_k - r_{k-1}>
+      // / t_prev, which tolerates the cycle's rounding; FP64: pcg.jl:49
+      double beta = t / t_prev;
+      if (mg->f32) {
+        double s = 0.0;
+        SEM_TRY(wdot_host(ctx, mg, g, mg->pq, mg->pz, &s));
+        beta = (t - s) / t_prev;
+      }
+      SEM_TRY(launch_axpby(st, n, 1.0, mg->pz, beta, mg->pp, mg->pp));
     }
+    if (mg->f32)
+      SEM_HIP(hipMemcpyAsync(mg->pq, mg->pr, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     SEM_TRY(apply(ctx, op, mg->pp, mg->pw));
     double pw = 0.0;
     SEM_TRY(wdot_host(ctx, mg, g, mg->pp, mg->pw, &pw));

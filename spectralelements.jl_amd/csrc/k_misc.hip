@@ -717,6 +717,14 @@ int launch_gsmap_masked(hipStream_t st, const sem_gsmap* map, const Layout& g, i
   return SEM_OK;
 }
 
+int launch_gsmap_masked(hipStream_t st, const sem_gsmap* map, const Layout& g, int bits,
+                        const float* u, float* y) {
+  hipLaunchKernelGGL((k_gsmap<float>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
+                     map->seg, u, y, (long)map->n_global, bits, g.L, g.NC);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
 int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y) {
   hipLaunchKernelGGL((k_gsmap<float>), dim3(nblk(map->n_global)), dim3(256), 0, st, map->perm,
                      map->seg, u, y, (long)map->n_global, -1, 1, 1);

@@ -942,7 +942,7 @@ extern "C" int sem_pcg_noalloc(sem_ctx* ctx, const sem_operator* op, int precond
   SEM_ARG(ctx && op && b && x, "NULL argument");
   SEM_ARG(op->desc.assemble, "sem_pcg_noalloc needs an assembled operator (desc.assemble = 1)");
   SEM_ARG(precond == SEM_PRECOND_IDENTITY || (precond == SEM_PRECOND_DIAG && d),
-          "sem_pcg_noalloc: SEM_PRECOND_DIAG needs the diagonal d");
+          "sem_pcg_noalloc: precond must be SEM_PRECOND_IDENTITY, or SEM_PRECOND_DIAG with d");
   SEM_ARG(x != b, "sem_pcg_noalloc: x must not alias b");
   SEM_ARG(maxiter >= 0, "sem_pcg_noalloc: maxiter < 0");
   SEM_ARG(tol == tol, "sem_pcg_noalloc: tol is NaN");

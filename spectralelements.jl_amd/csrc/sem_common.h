@@ -236,6 +236,8 @@ int launch_gsmap(hipStream_t st, const sem_gsmap* map, const double* u, double* 
 // QQ' followed by the Dirichlet mask of bits (u and y may alias)
 int launch_gsmap_masked(hipStream_t st, const sem_gsmap* map, const Layout& g, int bits,
                         const double* u, double* y);
+int launch_gsmap_masked(hipStream_t st, const sem_gsmap* map, const Layout& g, int bits,
+                        const float* u, float* y);
 int launch_gsmap_f32(hipStream_t st, const sem_gsmap* map, const float* u, float* y);
 int launch_mask(hipStream_t st, int64_t n, const double* M, const double* u, double* y);
 int launch_mask_bc(hipStream_t st, const Layout& g, int bits, const double* u, double* y);

@@ -616,14 +616,15 @@ struct MGParams
     npow::Cint
     safety::Cdouble
     max_levels::Cint
+    precision::Cint
 end
 mutable struct MGPrecond
     h::Ptr{Cvoid}
     op::SEMOperator
 end
 function MGPrecond(op::SEMOperator; degree = 3, ratio = 20.0, degree_coarse = 8, npow = 20,
-                   safety = 1.1, max_levels = 32)
-    p = MGParams(degree, ratio, degree_coarse, npow, safety, max_levels)
+                   safety = 1.1, max_levels = 32, precision = SEM_F64)
+    p = MGParams(degree, ratio, degree_coarse, npow, safety, max_levels, precision)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:sem_mg_create, libsemhip), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ref{MGParams}, Ref{Ptr{Cvoid}}), ctx(), op.h, p, h))

@@ -41,7 +41,8 @@ class sem_op_desc(C.Structure):
 
 class sem_mg_params(C.Structure):
     _fields_ = [("degree", C.c_int), ("ratio", C.c_double), ("degree_coarse", C.c_int),
-                ("npow", C.c_int), ("safety", C.c_double), ("max_levels", C.c_int)]
+                ("npow", C.c_int), ("safety", C.c_double), ("max_levels", C.c_int),
+                ("precision", C.c_int)]
 
 
 class sem_pcg_info(C.Structure):

@@ -745,14 +745,18 @@ class MG:
     element grid, then h-multigrid on bilinear elements, Chebyshev-Jacobi
     smoothing. ``mg(r)`` applies one V-cycle to an assembled residual;
     ``pcg(b, op, opM=mg)`` runs PCG with it on the GPU (sem_pcg_mg). The
-    operator must be FP64, assembled, with scalar nu and k."""
+    operator must be FP64, assembled, with scalar nu and k. precision='f32'
+    runs the V-cycle in FP32 (level operators, smoothing, transfers); the PCG
+    stays FP64 and takes the flexible (Polak-Ribiere) beta."""
 
     def __init__(self, op: SEMOperator, degree: int = 3, ratio: float = 20.0,
                  degree_coarse: int = 8, npow: int = 20, safety: float = 1.1,
-                 max_levels: int = 32):
+                 max_levels: int = 32, precision: str = "f64"):
         self.op = op
+        if precision not in ("f64", "f32"):
+            raise ValueError("MG precision: 'f64' or 'f32'")
         prm = L.sem_mg_params(int(degree), float(ratio), int(degree_coarse), int(npow),
-                              float(safety), int(max_levels))
+                              float(safety), int(max_levels), _PREC[precision])
         h = C.c_void_p()
         L.call("sem_mg_create", ctx(), op.h, C.byref(prm), C.byref(h))
         self.h = h

@@ -32,6 +32,7 @@ CASES = [
     (18, 18, 2, 3, (False, False), "DDDD"),
     (20, 7, 3, 2, (True, False), "DDDD"),
     (6, 4, 5, 4, (False, True), "DNDN"),
+    (40, 40, 1, 2, (False, False), "DDDD"),
 ]
 
 
@@ -86,3 +87,16 @@ def test_generic_pcg_matches_oracle(se, nr, ns, Ex, Ey, per, bc):
     assert _rel(x, xo) < 1e-9
     with pytest.raises(se.SemError, match="UNSUPPORTED"):
         se.SEMOperator(1.0, 1.0, m, M, precision="f32")
+
+
+def test_generic_block_beyond_lds_is_rejected(se):
+    """41 x 41 blocks need more than 64 KiB of LDS in the element kernel: the
+    operator and hlmz return SEM_ERR_UNSUPPORTED before any launch."""
+    _log("generic 41x41 rejected")
+    m = se.Mesh(41, 41, 1, 1, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    with pytest.raises(se.SemError, match="UNSUPPORTED"):
+        op(m.x)
+    with pytest.raises(se.SemError, match="UNSUPPORTED"):
+        se.hlmz(m.x, 1.0, 1.0, m)

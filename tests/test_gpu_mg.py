@@ -120,6 +120,36 @@ def test_pcg_mg_full_size_true_residual(se):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("n,E", [(8, 256), (16, 128)])
+def test_pcg_mg_fp32_cycle_full_size(se, n, E):
+    """The FP32 V-cycle (precision='f32') inside the FP64 PCG, at 256x256
+    N=7 and 128x128 N=15: the solve to 1e-10 relative meets the tolerance on
+    the true FP64 residual, takes at most two iterations more than with the
+    FP64 cycle, and agrees with the FP64-cycle solution to 1e-8; one FP32
+    V-cycle agrees with the FP64 one to FP32 accuracy (1e-5 relative)."""
+    import torch
+    from semhip import workloads as W
+    _log(f"pcg mg fp32 cycle {E}x{E} n={n}")
+    m = se.Mesh(n, n, E, E, (False, False), se.deform_sinsin(0.1))
+    M = se.generateMask("DDDD", m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    b = W.helmholtz_rhs(se, m, M, seed=2)
+    tol = 1e-10 * float(b.abs().max())
+    P64, P32 = se.MG(op), se.MG(op, precision="f32")
+    z64, z32 = P64(b), P32(b)
+    assert float((z32 - z64).abs().max()) <= 1e-5 * float(z64.abs().max())
+    i64, i32 = se.PCGInfo(), se.PCGInfo()
+    x64 = se.pcg(b, op, opM=P64, mult=m.mult, tol=tol, info=i64)
+    x32 = se.pcg(b, op, opM=P32, mult=m.mult, tol=tol, info=i32)
+    _log(f"  iterations fp64 cycle {i64.iters} ({i64.elapsed_ms:.1f} ms), "
+         f"fp32 cycle {i32.iters} ({i32.elapsed_ms:.1f} ms)")
+    assert i32.converged and i32.iters <= i64.iters + 2
+    assert float((b - op(x32)).abs().max()) <= 1.05 * tol
+    assert float((x32 - x64).abs().max()) <= 1e-8 * float(x64.abs().max())
+    del x64, x32, P64, P32
+    torch.cuda.synchronize()
+
+
 def test_mg_rejects_field_coefficients(se):
     _log("mg rejects field nu")
     m = se.Mesh(5, 5, 4, 4)
