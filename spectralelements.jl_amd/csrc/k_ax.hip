@@ -42,9 +42,25 @@ int dispatch(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunc
 
 AxPlan make_plan(const Layout& g, bool assemble, bool pcg, int rows) {
   AxPlan pl{};
-  pl.CX = SEM_AX_THREADS / g.P;
+  const int cx = SEM_AX_THREADS / g.P;
+  pl.S = assemble ? ax::rows_per_segment(g, cx, pcg, rows) : 1;
+  // The assembled FMA apply (P < 16; 16 runs the MFMA forms) takes two
+  // element rows side by side where one-row workgroups would be used, the
+  // PCG iteration where a two-row sweep would be (it also runs three waves
+  // per SIMD there instead of two). rows < 0 forces RY = -rows (test hook:
+  // 2 or 4; 2 in PCG mode).
+  pl.RY = 1;
+  if (assemble && g.P < 16) {
+    if (rows < 0) {
+      const int ry = -rows;
+      if ((ry == 2 || (ry == 4 && !pcg)) && cx % ry == 0) pl.RY = ry;
+    } else if (rows == 0 && pl.S == (pcg ? 2 : 1) && cx % 2 == 0 && g.Ey >= 2) {
+      pl.RY = 2;
+    }
+  }
+  if (pl.RY > 1) pl.S = pl.RY;
+  pl.CX = cx / pl.RY;
   pl.nstrips = (g.Ex + pl.CX - 1) / pl.CX;
-  pl.S = assemble ? ax::rows_per_segment(g, pl.CX, pcg, rows) : 1;
   pl.nsegs = (g.Ey + pl.S - 1) / pl.S;
   pl.nk1 = pl.nstrips * pl.nsegs;
   if (assemble) {

@@ -24,6 +24,10 @@
 //  * Nodes whose partner copies live in another workgroup (strip seams in x,
 //    segment seams in y, periodic wraps) are written unsummed and finished by
 //    k_seams in the same order. Every node copy is finalised exactly once.
+//  * The assembled FMA apply at P < 16 splits the workgroup into RY = 2 or 4
+//    groups of CX/RY elements on consecutive element rows side by side: the
+//    y-pairs between the rows are summed through LDS, each row writing its
+//    own copies, so a segment is RY rows without a longer sweep.
 //  * No atomics; the result is deterministic.
 #pragma once
 #include <algorithm>
@@ -172,11 +176,23 @@ __device__ __forceinline__ mf_f4 mf_contract(typename MfFrag<MF>::type ah,
 // MODE 0: element-local H u (no assembly); MODE 1: assembled + masked;
 // MODE 2: one fused PCG iteration body (see AxP). MF selects the MFMA
 // contraction path (T = float, P = 16 only).
-template <typename T, int P, int CX, int MODE, int MF = 0>
+// RY = 2 or 4 (assembled apply or PCG iteration on the FMA path; 4 in the
+// apply only): the workgroup's threads form
+// RY groups of CX / RY elements each, on RY consecutive element rows processed
+// side by side; the y-pairs between them are summed through LDS, so a segment
+// is RY element rows and 1 / RY of the segment boundaries of one-row
+// workgroups remain for k_seams, while every thread still issues one element
+// row's loads.
+template <typename T, int P, int CX, int MODE, int MF = 0, int RY = 1>
 __global__ void __launch_bounds__(CX* P)
     __attribute__((amdgpu_waves_per_eu((MF == 1 || MF == 2) ? 3 : 1))) k_ax(const AxP<T> prm) {
   constexpr int NR = CX * P;
   constexpr int RS = NR + 1;
+  constexpr int CXr = CX / RY;   // elements per strip row
+  constexpr int NRr = CXr * P;   // threads per element row
+  static_assert(RY == 1 || (MF == 0 && CX % RY == 0 &&
+                            ((RY == 2 && MODE != 0) || (RY == 4 && MODE == 1))),
+                "rows side by side: assembled FMA apply, strip width a multiple of RY");
   static_assert(MF == 0 || (P == 16 && NR == 256), "MFMA path: 16x16 blocks");
   static_assert(MF == 0 || MF == 6 || sizeof(T) == 4, "MFMA modes 1..5 take FP32 fields");
   static_assert(MF != 6 || sizeof(T) == 8, "MFMA mode 6 takes FP64 fields");
@@ -194,6 +210,8 @@ __global__ void __launch_bounds__(CX* P)
   // PCG mode: the max tree of the partials and the stop flag
   __shared__ double red2[MODE == 2 ? NR : 1];
   __shared__ int flag;
+  // RY > 1: each row's top line, then each row's bottom line
+  __shared__ T ybuf[RY > 1 ? 2 * NR : 1];
   // MFMA A-operands: D and D' (Dr = Ds); hi and lo parts (MF 1..4) or the
   // four K-blocks (MF 5, 6)
   using Frag = typename MfFrag<(MF >= 1 && MF <= 4) ? MF : 1>::type;
@@ -306,14 +324,23 @@ __global__ void __launch_bounds__(CX* P)
   const int Ex = prm.Ex, Ey = prm.Ey;
   const int L = P * Ex;
   const int NC = P * Ey;
-  const int ex0 = strip * CX;
-  const int ncx = min(CX, Ex - ex0);
-  const int e = tid / P;
-  const int a = tid - e * P;
-  const bool act = e < ncx;
-  const int i = ex0 * P + tid;
+  const int ex0 = strip * CXr;
+  const int ncx = min(CXr, Ex - ex0);
+  // row group hh of the workgroup (RY > 1), its thread tr = (e, a), and eo =
+  // the element's slot in the workgroup's LDS rows
+  const int hh = RY > 1 ? tid / NRr : 0;
+  const int tr = tid - hh * NRr;
+  const int e = tr / P;
+  const int a = tr - e * P;
+  const int eo = hh * CXr + e;
   const int ey0 = seg * prm.S;
   const int ey1 = min(Ey, ey0 + prm.S);
+  // this thread's element rows: the whole segment, or row ey0 + hh
+  const int eyA = RY > 1 ? ey0 + hh : ey0;
+  const int eyB = RY > 1 ? eyA + 1 : ey1;
+  const bool rowact = RY == 1 || eyA < Ey;
+  const bool act = e < ncx && rowact;
+  const int i = ex0 * P + tr;
   // Dr = Ds = derivMat(gausslobatto(P)) (mesh.jl:70-74, nr == ns) is a
   // compile-time table generated by the library's own host routine (the
   // mesh checks bit-equality at creation), so every coefficient is an
@@ -349,10 +376,12 @@ __global__ void __launch_bounds__(CX* P)
   const bool plain = prm.nu_f == nullptr && prm.k_f == nullptr;
 
   T carry = T(0);
-  for (int ey = ey0; ey < ey1; ++ey) {
+  T carry_hi = T(0);   // RY > 1: the copy of the top line from the row above
+  for (int ey = eyA; ey < eyB; ++ey) {
     const int jb = ey * P;
     const int base = i + L * jb;
-    const int bc_ = ic + L * jb;
+    // an idle row group (past Ey) loads the last element row
+    const int bc_ = ic + L * (rowact ? jb : (Ey - 1) * P);
     T uv[P], us[P];
     // the row's geometric factors, issued right after its other loads so
     // that all of them are in flight together (their use is two phases later)
@@ -482,13 +511,13 @@ __global__ void __launch_bounds__(CX* P)
     // C1: column owner (e, b'=a): ur(:, b') = Dr u(:, b')
       T v[P];
 #pragma unroll
-      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
+      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + eo * P + k];
 #pragma unroll
       for (int a2 = 0; a2 < P; ++a2) {
         T acc = T(0);
 #pragma unroll
         for (int k = 0; k < P; ++k) acc += Dr(a2 + P * k) * v[k];
-        s1[a * RS + e * P + a2] = acc;
+        s1[a * RS + eo * P + a2] = acc;
       }
     }
     __syncthreads();
@@ -605,13 +634,13 @@ __global__ void __launch_bounds__(CX* P)
     // C2: column owner: part1(:, b') = Dr' wr(:, b')
       T v[P];
 #pragma unroll
-      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + e * P + k];
+      for (int k = 0; k < P; ++k) v[k] = s0[a * RS + eo * P + k];
 #pragma unroll
       for (int a2 = 0; a2 < P; ++a2) {
         T acc = T(0);
 #pragma unroll
         for (int k = 0; k < P; ++k) acc += Dr(k + P * a2) * v[k];
-        s1[a * RS + e * P + a2] = acc;
+        s1[a * RS + eo * P + a2] = acc;
       }
     }
     __syncthreads();
@@ -662,17 +691,28 @@ __global__ void __launch_bounds__(CX* P)
     T* xb = (MF || kInPlace) ? xbuf : s0;
     if (xside >= 0) {
 #pragma unroll
-      for (int b = 0; b < P; ++b) xb[(xside * CX + e) * P + b] = h[b];
+      for (int b = 0; b < P; ++b) xb[(xside * CX + eo) * P + b] = h[b];
     }
     __syncthreads();
     if (xpair) {
       if (xside == 1) {
 #pragma unroll
-        for (int b = 0; b < P; ++b) h[b] = h[b] + xb[(e + 1) * P + b];
+        for (int b = 0; b < P; ++b) h[b] = h[b] + xb[(eo + 1) * P + b];
       } else {
 #pragma unroll
-        for (int b = 0; b < P; ++b) h[b] = xb[(CX + e - 1) * P + b] + h[b];
+        for (int b = 0; b < P; ++b) h[b] = xb[(CX + eo - 1) * P + b] + h[b];
       }
+    }
+    if constexpr (RY > 1) {
+      // y-pairs of the rows: every row publishes its top and bottom lines
+      // (x-pairs already summed) and takes the partner copies: carry from the
+      // row below, carry_hi from the row above; each row writes only its own
+      // copies
+      ybuf[hh * NRr + tr] = h[P - 1];
+      ybuf[NR + hh * NRr + tr] = h[0];
+      __syncthreads();
+      if (hh > 0) carry = ybuf[(hh - 1) * NRr + tr];
+      if (hh < RY - 1) carry_hi = ybuf[NR + (hh + 1) * NRr + tr];
     }
     if (act) {
       const int bc = prm.bc;
@@ -688,11 +728,11 @@ __global__ void __launch_bounds__(CX* P)
             if (ey > ey0) {
               if (xseam) {
                 prm.y[idx] = h[0];
-                prm.y[idx - L] = carry;
+                if (RY == 1) prm.y[idx - L] = carry;
               } else {
                 const T s = carry + h[0];
                 fin(idx, j, s, uv[0]);
-                fin(idx - L, j - 1, s, carry_u);
+                if (RY == 1) fin(idx - L, j - 1, s, carry_u);
               }
             } else {
               prm.y[idx] = h[0];  // y seam: finished by k_seams
@@ -705,8 +745,13 @@ __global__ void __launch_bounds__(CX* P)
           const bool ysh = ey < Ey - 1 || prm.per_y;
           if (ysh) {
             if (ey < ey1 - 1) {
-              carry = h[P - 1];
-              carry_u = uv[P - 1];
+              if constexpr (RY > 1) {
+                if (xseam) prm.y[idx] = h[P - 1];
+                else fin(idx, j, h[P - 1] + carry_hi, uv[P - 1]);
+              } else {
+                carry = h[P - 1];
+                carry_u = uv[P - 1];
+              }
             } else {
               prm.y[idx] = h[P - 1];  // y seam
             }
@@ -854,6 +899,17 @@ void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mod
   else if (mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, MF>), grid, block, 0, st, prm);
   else hipLaunchKernelGGL((k_ax<T, P, CX, 0, MF>), grid, block, 0, st, prm);
 }
+template <typename T, int P, int CX>
+void launch_k1_rows(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int ry, int mode) {
+  if constexpr (P < 16) {
+    if constexpr (CX % 4 == 0)
+      if (ry == 4 && mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 4>), grid, block, 0, st, prm);
+    if constexpr (CX % 2 == 0) {
+      if (ry == 2 && mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2>), grid, block, 0, st, prm);
+      if (ry == 2 && mode == 2) hipLaunchKernelGGL((k_ax<T, P, CX, 2, 0, 2>), grid, block, 0, st, prm);
+    }
+  }
+}
 
 template <typename T, int P>
 int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const PcgLaunch* pc) {
@@ -880,7 +936,9 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   // when the operator is created) selects the FMA path.
   int mf = a.mfma;
   if (mf == 0 && P == 16 && CX * P == 256 && !a.fma16) mf = sizeof(T) == 8 ? 6 : 5;
-  if (mf == 0) {
+  if (mf == 0 && pl.RY > 1) {
+    launch_k1_rows<T, P, CX>(st, grid, block, prm, pl.RY, mode);
+  } else if (mf == 0) {
     launch_k1<T, P, CX, 0>(st, grid, block, prm, mode);
   } else {
     if constexpr (P == 16 && CX * P == 256) {
@@ -912,7 +970,7 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
     const int nys = pl.nsegs - 1 + (g.per_y ? 1 : 0);
     const int nxs = pl.nstrips - 1 + (g.per_x ? 1 : 0);
     const dim3 sgrid((std::max(g.L, g.NC) + kSeamBlock - 1) / kSeamBlock, nys + nxs);
-    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P, CX * P,
+    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P, pl.CX * P,
                        pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys);
   }
   SEM_HIP(hipGetLastError());

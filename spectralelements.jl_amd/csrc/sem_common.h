@@ -111,7 +111,7 @@ struct sem_operator {
   // by 2^e (e chosen per array so the largest magnitude is near 2^14);
   // gscale = 2^-e
   float gscale[4] = {1.f, 1.f, 1.f, 1.f};
-  int ax_rows = 0;    // SEM_AX_ROWS at creation (0: the heuristic)
+  int ax_rows = 0;    // SEM_AX_ROWS at creation (0: the heuristic; -2, -4: rows side by side)
   int ax_fma16 = 0;   // SEM_AX_FMA16=1 at creation
   // iterations of the last fresh solve with this operator: the first launch
   // batch of the next one (repeated solves: time steps, adjoints, FD)
@@ -155,7 +155,8 @@ struct ApplyArgs {
   // mfma 1, 2: G11, G12, G22, B are FP16 arrays; value = stored * gscale[q]
   float gscale[4];
   // test hooks, read from the environment when the operator is created
-  // (SEM_AX_ROWS: element rows per workgroup; SEM_AX_FMA16=1: FMA contractions
+  // (SEM_AX_ROWS: element rows swept per workgroup, or -2 / -4 rows side by
+  // side; SEM_AX_FMA16=1: FMA contractions
   // for 16x16 blocks instead of the exact FP64/FP32 MFMA forms)
   int rows;
   int fma16;
@@ -194,10 +195,12 @@ struct PcgDev {
 
 // Launch geometry of the fused apply: nk1 strip workgroups, nk2 seam blocks.
 struct AxPlan {
-  int CX, nstrips, S, nsegs, nk1, nk2;
+  // CX: elements per strip row; RY: element rows a workgroup processes side
+  // by side (1, or 2 for the assembled FMA apply); S: element rows per segment
+  int CX, RY, nstrips, S, nsegs, nk1, nk2;
   long nxitems, total;
 };
-// rows > 0 forces the element rows per workgroup
+// rows > 0 forces rows swept one after another per workgroup (RY = 1)
 AxPlan make_plan(const Layout& g, bool assemble, bool pcg, int rows = 0);
 
 struct PcgLaunch {

@@ -146,7 +146,9 @@ bool fused_mesh(const sem_mesh* m) {
 void read_ax_hooks(int* rows, int* fma16) {
   const char* r = std::getenv("SEM_AX_ROWS");
   const char* f = std::getenv("SEM_AX_FMA16");
-  *rows = r ? std::max(0, std::atoi(r)) : 0;
+  // rows > 0: rows swept per workgroup; -2 / -4: rows side by side
+  *rows = r ? std::atoi(r) : 0;
+  if (*rows < 0 && *rows != -2 && *rows != -4) *rows = 0;
   *fma16 = (f && std::atoi(f) == 1) ? 1 : 0;
 }
 

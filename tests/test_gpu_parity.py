@@ -134,7 +134,7 @@ def test_abu_matches_oracle(se):
 
 
 APPLY_CASES = [
-    # n, Ex, Ey, per, bc, rows per segment
+    # n, Ex, Ey, per, bc, rows per segment (> 0: swept one after another)
     (8, 70, 5, (False, False), "DDDD", 1),
     (8, 70, 5, (False, False), "DDDD", 2),
     (8, 70, 5, (False, False), "DNDN", 3),
@@ -149,6 +149,19 @@ APPLY_CASES = [
     (10, 30, 5, (False, False), "DDDD", 2),
     (5, 1, 3, (True, True), "NNNN", 2),
     (7, 2, 1, (True, True), "NNNN", 1),
+    # 0: the default plan (element rows side by side where the strip allows);
+    # -2 / -4: two / four rows side by side, forced
+    (8, 70, 5, (False, False), "DDDD", 0),
+    (8, 70, 5, (True, True), "NNNN", -2),
+    (8, 33, 6, (False, True), "DDNN", -4),
+    (8, 17, 3, (True, False), "NNDD", -4),
+    (8, 16, 2, (True, True), "NNNN", -4),
+    (4, 70, 3, (True, True), "NNNN", -4),
+    (2, 300, 3, (False, False), "DDDD", 0),
+    (7, 40, 9, (False, False), "DNDN", 0),
+    (6, 22, 7, (True, False), "DDDD", 0),
+    (9, 29, 5, (False, True), "NNDD", -4),
+    (14, 20, 3, (True, True), "NNNN", 0),
 ]
 
 
@@ -380,6 +393,10 @@ PCG_SEAM_CASES = [
     (8, 40, 7, (False, True), "DDNN", 4, "identity"),
     (6, 50, 4, (True, True), "NNNN", 1, "jacobi"),
     (16, 20, 5, (False, False), "DDDD", 2, "jacobi"),
+    # -2: two element rows side by side per workgroup
+    (8, 70, 6, (True, False), "NNDD", -2, "jacobi"),
+    (8, 40, 7, (False, True), "DDNN", -2, "identity"),
+    (6, 50, 5, (True, True), "NNNN", -2, "jacobi"),
 ]
 
 
