@@ -85,6 +85,12 @@ struct AxP {
   // leaves x alone)
   T* __restrict__ x;
   PcgDev pcg;
+  // fused epilogue (EPI 1, 2; see ApplyEpi): er, ex and y are written, u is
+  // only read, and none of them alias
+  T* er;
+  T* ex;
+  const T* edinv;
+  T ec1, ec2;
 };
 
 // ---- MFMA contractions for 16x16 element blocks (MF != 0) ----------------
@@ -183,7 +189,7 @@ __device__ __forceinline__ mf_f4 mf_contract(typename MfFrag<MF>::type ah,
 // is RY element rows and 1 / RY of the segment boundaries of one-row
 // workgroups remain for k_seams, while every thread still issues one element
 // row's loads.
-template <typename T, int P, int CX, int MODE, int MF = 0, int RY = 1>
+template <typename T, int P, int CX, int MODE, int MF = 0, int RY = 1, int EPI = 0>
 __global__ void __launch_bounds__(CX* P)
     __attribute__((amdgpu_waves_per_eu((MF == 1 || MF == 2) ? 3 : 1))) k_ax(const AxP<T> prm) {
   constexpr int NR = CX * P;
@@ -194,6 +200,7 @@ __global__ void __launch_bounds__(CX* P)
                             ((RY == 2 && MODE != 0) || (RY == 4 && MODE == 1))),
                 "rows side by side: assembled FMA apply, strip width a multiple of RY");
   static_assert(MF == 0 || (P == 16 && NR == 256), "MFMA path: 16x16 blocks");
+  static_assert(EPI == 0 || (MODE == 1 && MF == 0 && RY == 2), "fused epilogue: two-row apply");
   static_assert(MF == 0 || MF == 6 || sizeof(T) == 4, "MFMA modes 1..5 take FP32 fields");
   static_assert(MF != 6 || sizeof(T) == 8, "MFMA mode 6 takes FP64 fields");
   // The FMA path (MF == 0) runs C1, R2 and C2 in place: in each of them a
@@ -644,6 +651,13 @@ __global__ void __launch_bounds__(CX* P)
       }
     }
     __syncthreads();
+    // fused epilogue operands of this thread's copies, in flight while H
+    // and the pair sums are formed (clamped rows for inactive threads)
+    EpiIn<T> ein[EPI ? P : 1];
+    if constexpr (EPI != 0) {
+#pragma unroll
+      for (int b = 0; b < P; ++b) ein[b] = epi_load<T, EPI>(bc_ + L * b, prm.er, prm.ex, prm.edinv);
+    }
     // R3: H = nu (part1 + part2) + k (B u)
     T h[P];
     if (plain) {
@@ -716,8 +730,14 @@ __global__ void __launch_bounds__(CX* P)
     }
     if (act) {
       const int bc = prm.bc;
-      // final (masked) write of a node copy
-      auto fin = [&](int idx, int j, T val, T) { prm.y[idx] = val * mval<T>(i, j, L, NC, bc); };
+      // final (masked) write of a node copy; with a fused epilogue the
+      // final value feeds it instead (uin: the copy's input value; eb: its
+      // slot among this thread's preloaded epilogue operands)
+      auto fin = [&](int idx, int j, T val, T uin, int eb) {
+        const T wv = val * mval<T>(i, j, L, NC, bc);
+        if constexpr (EPI == 0) prm.y[idx] = wv;
+        else epi_store<T, EPI>(idx, wv, uin, ein[eb], prm.er, prm.ex, prm.ec1, prm.ec2, prm.y);
+      };
 #pragma unroll
       for (int b = 0; b < P; ++b) {
         const int j = jb + b;
@@ -731,15 +751,15 @@ __global__ void __launch_bounds__(CX* P)
                 if (RY == 1) prm.y[idx - L] = carry;
               } else {
                 const T s = carry + h[0];
-                fin(idx, j, s, uv[0]);
-                if (RY == 1) fin(idx - L, j - 1, s, carry_u);
+                fin(idx, j, s, uv[0], 0);
+                if (RY == 1) fin(idx - L, j - 1, s, carry_u, 0);
               }
             } else {
               prm.y[idx] = h[0];  // y seam: finished by k_seams
             }
           } else {
             if (xseam) prm.y[idx] = h[0];
-            else fin(idx, j, h[0], uv[0]);
+            else fin(idx, j, h[0], uv[0], 0);
           }
         } else if (b == P - 1) {
           const bool ysh = ey < Ey - 1 || prm.per_y;
@@ -747,7 +767,7 @@ __global__ void __launch_bounds__(CX* P)
             if (ey < ey1 - 1) {
               if constexpr (RY > 1) {
                 if (xseam) prm.y[idx] = h[P - 1];
-                else fin(idx, j, h[P - 1] + carry_hi, uv[P - 1]);
+                else fin(idx, j, h[P - 1] + carry_hi, uv[P - 1], P - 1);
               } else {
                 carry = h[P - 1];
                 carry_u = uv[P - 1];
@@ -757,11 +777,11 @@ __global__ void __launch_bounds__(CX* P)
             }
           } else {
             if (xseam) prm.y[idx] = h[P - 1];
-            else fin(idx, j, h[P - 1], uv[P - 1]);
+            else fin(idx, j, h[P - 1], uv[P - 1], P - 1);
           }
         } else {
           if (xseam) prm.y[idx] = h[b];
-          else fin(idx, j, h[b], uv[b]);
+          else fin(idx, j, h[b], uv[b], b);
         }
       }
     }
@@ -784,12 +804,34 @@ __global__ void __launch_bounds__(CX* P)
 // no races. The kernel is latency-bound (a few MB in thousands of short
 // blocks): 128-thread blocks measured best (1 item per thread beat 4).
 constexpr int kSeamBlock = 128;
-template <typename T>
+// EPI: the fused epilogue of the apply (see ApplyEpi) on the copies finished
+// here; u is the apply's input (the old direction for EPI 1).
+template <typename T, int EPI = 0>
 __global__ void __launch_bounds__(kSeamBlock)
-k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs,
-        int per_x, int per_y, int bc, int nys) {
+k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int per_x, int per_y,
+        int bc, int nys, const T* u, T* er, T* ex, const T* edinv, T ec1, T ec2) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  auto fin = [&](int i, int j, T s) { y[i + (long)L * j] = s * mval<T>(i, j, L, NC, bc); };
+  // a copy's epilogue operands (and input value) are loaded together with
+  // the seam copies, then the update is stored from the final value
+  struct Copy {
+    long t;
+    EpiIn<T> e;
+    T u;
+  };
+  auto pre = [&](int i, int j) {
+    Copy c{};
+    c.t = i + (long)L * j;
+    if constexpr (EPI != 0) {
+      c.e = epi_load<T, EPI>(c.t, er, ex, edinv);
+      if constexpr (EPI == 1) c.u = u[c.t];
+    }
+    return c;
+  };
+  auto fin = [&](int i, int j, const Copy& c, T s) {
+    const T wv = s * mval<T>(i, j, L, NC, bc);
+    if constexpr (EPI == 0) y[c.t] = wv;
+    else epi_store<T, EPI>(c.t, wv, c.u, c.e, er, ex, ec1, ec2, y);
+  };
   if ((int)blockIdx.y < nys) {
     const int g = blockIdx.y;
     int jL, jR;
@@ -806,9 +848,10 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
     const bool xs_row = (r == 0 && i > 0) || (r == CXP - 1 && i < L - 1) ||
                         (per_x && (i == 0 || i == L - 1));
     if (xs_row) return;   // summed with the four-copy rule by the x-seam items
-    const T sm = y[i + (long)L * jL] + y[i + (long)L * jR];
-    fin(i, jL, sm);
-    fin(i, jR, sm);
+    const Copy cL = pre(i, jL), cR = pre(i, jR);
+    const T sm = y[cL.t] + y[cR.t];
+    fin(i, jL, cL, sm);
+    fin(i, jR, cR, sm);
   } else {
     const int s = blockIdx.y - nys;
     int iL, iR;
@@ -825,17 +868,19 @@ k_seams(T* __restrict__ y, int L, int NC, int P, int CXP, int nstrips, int SP, i
     if (b == 0 && (j > 0 || per_y)) return;  // summed by the item of the row below
     if (b == P - 1 && (j < NC - 1 || per_y)) {
       const int jU = (j + 1 == NC) ? 0 : j + 1;
-      const T sl = y[iL + (long)L * j] + y[iR + (long)L * j];
-      const T su = y[iL + (long)L * jU] + y[iR + (long)L * jU];
+      const Copy c0 = pre(iL, j), c1 = pre(iR, j), c2 = pre(iL, jU), c3 = pre(iR, jU);
+      const T sl = y[c0.t] + y[c1.t];
+      const T su = y[c2.t] + y[c3.t];
       const T sm = sl + su;
-      fin(iL, j, sm);
-      fin(iR, j, sm);
-      fin(iL, jU, sm);
-      fin(iR, jU, sm);
+      fin(iL, j, c0, sm);
+      fin(iR, j, c1, sm);
+      fin(iL, jU, c2, sm);
+      fin(iR, jU, c3, sm);
     } else {
-      const T sm = y[iL + (long)L * j] + y[iR + (long)L * j];
-      fin(iL, j, sm);
-      fin(iR, j, sm);
+      const Copy c0 = pre(iL, j), c1 = pre(iR, j);
+      const T sm = y[c0.t] + y[c1.t];
+      fin(iL, j, c0, sm);
+      fin(iR, j, c1, sm);
     }
   }
 }
@@ -899,6 +944,15 @@ void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mod
   else if (mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, MF>), grid, block, 0, st, prm);
   else hipLaunchKernelGGL((k_ax<T, P, CX, 0, MF>), grid, block, 0, st, prm);
 }
+// the two-row apply with a fused epilogue (kind 1 or 2): P = 2, 4, 8, the
+// multigrid levels of N = 7 and 15
+template <typename T, int P, int CX>
+void launch_k1_epi(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int kind) {
+  if constexpr (CX % 2 == 0 && (P == 2 || P == 4 || P == 8)) {
+    if (kind == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2, 1>), grid, block, 0, st, prm);
+    if (kind == 2) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2, 2>), grid, block, 0, st, prm);
+  }
+}
 template <typename T, int P, int CX>
 void launch_k1_rows(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int ry, int mode) {
   if constexpr (P < 16) {
@@ -936,7 +990,20 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   // when the operator is created) selects the FMA path.
   int mf = a.mfma;
   if (mf == 0 && P == 16 && CX * P == 256 && !a.fma16) mf = sizeof(T) == 8 ? 6 : 5;
-  if (mf == 0 && pl.RY > 1) {
+  const int epi = a.epi.kind;
+  if (epi != 0) {
+    if (!(epi == 1 || epi == 2) || mf != 0 || mode != 1 || pl.RY != 2 ||
+        !(P == 2 || P == 4 || P == 8) || !a.epi.r || (epi == 1 && (!a.epi.x || !a.epi.dinv))) {
+      set_error("fused apply epilogue %d: unsupported launch", epi);
+      return SEM_ERR_UNSUPPORTED;
+    }
+    prm.er = static_cast<T*>(a.epi.r);
+    prm.ex = static_cast<T*>(a.epi.x);
+    prm.edinv = static_cast<const T*>(a.epi.dinv);
+    prm.ec1 = T(a.epi.c1);
+    prm.ec2 = T(a.epi.c2);
+    launch_k1_epi<T, P, CX>(st, grid, block, prm, epi);
+  } else if (mf == 0 && pl.RY > 1) {
     launch_k1_rows<T, P, CX>(st, grid, block, prm, pl.RY, mode);
   } else if (mf == 0) {
     launch_k1<T, P, CX, 0>(st, grid, block, prm, mode);
@@ -970,8 +1037,18 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
     const int nys = pl.nsegs - 1 + (g.per_y ? 1 : 0);
     const int nxs = pl.nstrips - 1 + (g.per_x ? 1 : 0);
     const dim3 sgrid((std::max(g.L, g.NC) + kSeamBlock - 1) / kSeamBlock, nys + nxs);
-    hipLaunchKernelGGL((k_seams<T>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P, pl.CX * P,
-                       pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys);
+    if (epi == 1)
+      hipLaunchKernelGGL((k_seams<T, 1>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
+                         pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
+                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2);
+    else if (epi == 2)
+      hipLaunchKernelGGL((k_seams<T, 2>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
+                         pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
+                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2);
+    else
+      hipLaunchKernelGGL((k_seams<T, 0>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
+                         pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
+                         nullptr, nullptr, nullptr, nullptr, T(0), T(0));
   }
   SEM_HIP(hipGetLastError());
   return SEM_OK;

@@ -12,6 +12,7 @@
 // output node, the element's interpolation rows from a small table.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -56,7 +57,7 @@ __global__ void k_cheb_step(const T* __restrict__ w, T* __restrict__ r,
   if (t >= n) return;
   const T rv = r[t] - w[t];
   r[t] = rv;
-  const T dv = T(c1) * d[t] + T(c2) * (dinv[t] * rv);
+  const T dv = cheb_dir(T(c1), d[t], T(c2), dinv[t], rv);
   d[t] = dv;
   x[t] = x[t] + dv;
 }
@@ -75,64 +76,109 @@ struct Xfer {
   int per_x, per_y;
 };
 
-// e = P xc (per fine element: Jx xc Jy'), x += e
+// e = P xc (per fine element: Jx xc Jy'), x += e. Row owner: thread i of
+// grid row eyf holds fine row i of fine element row eyf; it forms the
+// x-contraction s(bb) = sum_aa Jx(a, aa) xc(aa, bb) once and then every
+// column b of the element, so each coarse value is loaded pc / pf times per
+// fine node and there is one integer division per thread. Same operation
+// order as the one-node-per-thread form: v = sum_bb Jy(b, bb) s(bb).
+constexpr int kXferMaxP = 16;
 template <typename T>
-__global__ void k_prolong(const T* __restrict__ xc, const T* __restrict__ J, T* __restrict__ e,
-                          T* __restrict__ x, Xfer g, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % g.Lf), j = (int)(t / g.Lf);
+__global__ void __launch_bounds__(256) k_prolong(const T* __restrict__ xc, const T* __restrict__ J,
+                                                  T* __restrict__ e, T* __restrict__ x, Xfer g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.Lf) return;
+  const int eyf = blockIdx.y;
   const int exf = i / g.pf, a = i - exf * g.pf;
-  const int eyf = j / g.pf, b = j - eyf * g.pf;
   const int sx = exf & g.hx, sy = eyf & g.hy;
   const int exc = exf >> g.hx, eyc = eyf >> g.hy;
   const int blk = g.pf * g.pc;
   const T* jx = J + sx * blk + a * g.pc;
-  const T* jy = J + (g.hx ? 2 : 1) * blk + sy * blk + b * g.pc;
+  const T* jy = J + (g.hx ? 2 : 1) * blk + sy * blk;
   const long c0 = (long)exc * g.pc + (long)g.Lc * ((long)eyc * g.pc);
-  T v = T(0);
-  for (int bb = 0; bb < g.pc; ++bb) {
-    const T wy = jy[bb];
-    T s = T(0);
-    for (int aa = 0; aa < g.pc; ++aa) s += jx[aa] * xc[c0 + aa + (long)g.Lc * bb];
-    v += wy * s;
+  T sv[kXferMaxP];
+#pragma unroll
+  for (int bb = 0; bb < kXferMaxP; ++bb) {
+    if (bb < g.pc) {
+      T sacc = T(0);
+      for (int aa = 0; aa < g.pc; ++aa) sacc += jx[aa] * xc[c0 + aa + (long)g.Lc * bb];
+      sv[bb] = sacc;
+    }
   }
-  e[t] = v;
-  x[t] = x[t] + v;
+  const long f0 = i + (long)g.Lf * ((long)eyf * g.pf);
+  for (int b = 0; b < g.pf; ++b) {
+    T v = T(0);
+#pragma unroll
+    for (int bb = 0; bb < kXferMaxP; ++bb)
+      if (bb < g.pc) v += jy[b * g.pc + bb] * sv[bb];
+    const long t = f0 + (long)g.Lf * b;
+    e[t] = v;
+    x[t] = x[t] + v;
+  }
 }
 
 // rc (element-local) = Jx' (mult_f .* rf) Jy summed over the fine nodes of
-// the coarse element; assembled and masked afterwards by the coarse QQ'
+// the coarse element; assembled and masked afterwards by the coarse QQ'.
+// A workgroup takes one coarse element row (grid row) and the fine rows of
+// ce whole coarse elements: it stages those rows of the element row's fine
+// columns in LDS with the x multiplicity applied (coalesced loads; the
+// multiplicities are 1 or 1/2, so the product is exact), then forms the
+// x-contractions s(a, c) = sum_(sx, af) Jx(af, a) rf(af, c) once per coarse
+// row and fine column, and each coarse node v = sum_c wy(c, b) s(a, c) with
+// wy = Jy my: the operation order of the one-node-per-thread form, so the
+// bits are the same.
 template <typename T>
-__global__ void k_restrict(const T* __restrict__ rf, const T* __restrict__ J, T* __restrict__ rc,
-                           Xfer g, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % g.Lc), j = (int)(t / g.Lc);
-  const int exc = i / g.pc, a = i - exc * g.pc;
-  const int eyc = j / g.pc, b = j - eyc * g.pc;
-  const int blk = g.pf * g.pc;
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ rf, const T* __restrict__ J,
+                                                   T* __restrict__ rc, Xfer g) {
+  extern __shared__ unsigned char xfer_lds[];
   const int nsx = g.hx ? 2 : 1, nsy = g.hy ? 2 : 1;
-  const T* Jy0 = J + nsx * blk;
-  T v = T(0);
-  for (int sy = 0; sy < nsy; ++sy) {
-    const int eyf = (eyc << g.hy) + sy;
-    for (int bf = 0; bf < g.pf; ++bf) {
-      const int jf = eyf * g.pf + bf;
-      const T wy = Jy0[sy * blk + bf * g.pc + b] * T(mult1(jf, g.pf, g.NCf, g.per_y));
-      T s = T(0);
-      for (int sx = 0; sx < nsx; ++sx) {
-        const int exf = (exc << g.hx) + sx;
-        for (int af = 0; af < g.pf; ++af) {
-          const int ifn = exf * g.pf + af;
-          s += J[sx * blk + af * g.pc + a] * T(mult1(ifn, g.pf, g.Lf, g.per_x)) *
-               rf[ifn + (long)g.Lf * jf];
-        }
-      }
-      v += wy * s;
-    }
+  const int fw = nsx * g.pf;          // fine rows per coarse element
+  const int ce = 256 / fw;            // coarse elements per workgroup
+  const int FR = ce * fw;             // fine rows staged
+  const int nfc = nsy * g.pf;         // fine columns of the coarse element row
+  const int rows = ce * g.pc;         // coarse rows of the workgroup
+  const int blk = g.pf * g.pc;
+  T* tile = reinterpret_cast<T*>(xfer_lds);   // [nfc][FR]
+  T* sx_ = tile + FR * nfc;                   // [nfc][rows]
+  T* jx = sx_ + rows * nfc;                   // [nsx * blk]: J's x blocks
+  T* wy = jx + nsx * blk;                     // [nfc][pc]: Jy times the y multiplicity
+  const int eyc = blockIdx.y;
+  const int i0 = blockIdx.x * FR;
+  const int jf0 = eyc * nfc;
+  for (int q = threadIdx.x; q < FR * nfc; q += blockDim.x) {
+    const int c = q / FR, r = q - c * FR;
+    const int ifn = i0 + r;
+    T v = T(0);
+    if (ifn < g.Lf) v = T(mult1(ifn, g.pf, g.Lf, g.per_x)) * rf[ifn + (long)g.Lf * (jf0 + c)];
+    tile[c * FR + r] = v;
   }
-  rc[t] = v;
+  for (int q = threadIdx.x; q < nsx * blk; q += blockDim.x) jx[q] = J[q];
+  for (int q = threadIdx.x; q < nfc * g.pc; q += blockDim.x) {
+    const int c = q / g.pc, b = q - c * g.pc;
+    const int sy = c / g.pf, bf = c - sy * g.pf;
+    const int jf = ((eyc << g.hy) + sy) * g.pf + bf;
+    wy[q] = J[nsx * blk + sy * blk + bf * g.pc + b] * T(mult1(jf, g.pf, g.NCf, g.per_y));
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < rows * nfc; q += blockDim.x) {
+    const int c = q / rows, rl = q - c * rows;
+    const int exl = rl / g.pc, a = rl - exl * g.pc;
+    const T* trow = tile + c * FR + exl * fw;
+    T sacc = T(0);
+    for (int sx = 0; sx < nsx; ++sx)
+      for (int af = 0; af < g.pf; ++af) sacc += jx[sx * blk + af * g.pc + a] * trow[sx * g.pf + af];
+    sx_[c * rows + rl] = sacc;
+  }
+  __syncthreads();
+  const int exc0 = i0 / fw;
+  for (int o = threadIdx.x; o < rows * g.pc; o += blockDim.x) {
+    const int b = o / rows, rl = o - b * rows;
+    const int ic = exc0 * g.pc + rl;
+    if (ic >= g.Lc) continue;
+    T v = T(0);
+    for (int c = 0; c < nfc; ++c) v += wy[c * g.pc + b] * sx_[c * rows + rl];
+    rc[ic + (long)g.Lc * (eyc * g.pc + b)] = v;
+  }
 }
 
 // coarse coordinates: the fine field interpolated at the coarse nodes; Jc is
@@ -239,11 +285,17 @@ struct sem_mg {
     const sem_operator* op32 = nullptr;
     float *fx = nullptr, *fb = nullptr, *fr = nullptr, *fd = nullptr, *fw = nullptr,
           *fe = nullptr, *fdinv = nullptr, *fJ = nullptr;
+    // the level's applies carry the smoother's vector updates as a fused
+    // epilogue (FP64 / FP32 cycle)
+    bool fuse64 = false, fuse32 = false;
   };
   sem_mg_params prm{};
   int device = 0;
   const sem_operator* op = nullptr;
   bool f32 = false;
+  // smoother updates fused into the level applies (SEM_MG_FUSE=0 at
+  // creation turns it off: a test hook)
+  bool fuse = true;
   std::vector<Level> lv;
   // PCG vectors on the fine level (pq: the previous residual of the flexible
   // recurrence), reduction partials and the scalar slot
@@ -290,16 +342,29 @@ template <typename T>
 struct LvBuf {
   T *x, *b, *r, *d, *w, *e, *dinv, *J;
   const sem_operator* op;
+  bool fuse;
 };
 template <typename T>
 LvBuf<T> bufs(sem_mg::Level& L);
 template <>
 LvBuf<double> bufs<double>(sem_mg::Level& L) {
-  return {L.x, L.b, L.r, L.d, L.w, L.e, L.dinv, L.J, L.op};
+  return {L.x, L.b, L.r, L.d, L.w, L.e, L.dinv, L.J, L.op, L.fuse64};
 }
 template <>
 LvBuf<float> bufs<float>(sem_mg::Level& L) {
-  return {L.fx, L.fb, L.fr, L.fd, L.fw, L.fe, L.fdinv, L.fJ, L.op32};
+  return {L.fx, L.fb, L.fr, L.fd, L.fw, L.fe, L.fdinv, L.fJ, L.op32, L.fuse32};
+}
+// the fused Chebyshev step leaves the new direction in w: the two buffers
+// trade roles
+template <typename T>
+void swap_dw(sem_mg::Level& L);
+template <>
+void swap_dw<double>(sem_mg::Level& L) {
+  std::swap(L.d, L.w);
+}
+template <>
+void swap_dw<float>(sem_mg::Level& L) {
+  std::swap(L.fd, L.fw);
 }
 
 // host value of a device reduction result
@@ -327,6 +392,23 @@ int apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y) {
   return sem_operator_apply(ctx, op, u, y);
 }
 
+// r -= A v (fused into the apply's final writes where the level allows;
+// w is scratch either way)
+template <typename T>
+int residual_update(sem_ctx* ctx, const LvBuf<T>& L, const T* v) {
+  if (L.fuse) {
+    ApplyEpi e;
+    e.kind = 2;
+    e.r = L.r;
+    return operator_apply_epi(ctx, L.op, v, L.w, e);
+  }
+  SEM_TRY(apply(ctx, L.op, v, L.w));
+  const long n = (long)L.op->mesh->nl;
+  hipLaunchKernelGGL((k_sub<T>), dim3(nblk(n)), dim3(256), 0, ctx->stream, L.r, L.w, n);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
 // Chebyshev iteration of degree deg for D^-1 A on [lmax / ratio, lmax]; from
 // zero (x = 0, r = b) or from (x, r). With want_r the residual of the final
 // x is formed in r (one more apply).
@@ -334,7 +416,7 @@ template <typename T>
 int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& Lv, const T* b, int from_zero, int deg,
           bool want_r) {
   hipStream_t st = ctx->stream;
-  const LvBuf<T> L = bufs<T>(Lv);
+  LvBuf<T> L = bufs<T>(Lv);
   const long n = (long)Lv.mesh->nl;
   const double lmax = Lv.lmax, lmin = Lv.lmax / mg->prm.ratio;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
@@ -344,18 +426,62 @@ int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& Lv, const T* b, int from_zero
                      L.x, theta, from_zero, n);
   SEM_HIP(hipGetLastError());
   for (int i = 1; i < deg; ++i) {
-    SEM_TRY(apply(ctx, L.op, L.d, L.w));
     const double rho_n = 1.0 / (2.0 * sigma - rho);
-    hipLaunchKernelGGL((k_cheb_step<T>), dim3(nblk(n)), dim3(256), 0, st, L.w, L.r, L.dinv, L.d,
-                       L.x, rho_n * rho, 2.0 * rho_n / delta, n);
-    SEM_HIP(hipGetLastError());
+    if (L.fuse) {
+      // w = A d with r -= w; d' = c1 d + c2 dinv .* r; x += d' fused into
+      // its final writes (d' lands in w, which becomes the direction)
+      ApplyEpi e;
+      e.kind = 1;
+      e.r = L.r;
+      e.x = L.x;
+      e.dinv = L.dinv;
+      e.c1 = rho_n * rho;
+      e.c2 = 2.0 * rho_n / delta;
+      SEM_TRY(operator_apply_epi(ctx, L.op, L.d, L.w, e));
+      swap_dw<T>(Lv);
+      L = bufs<T>(Lv);
+    } else {
+      SEM_TRY(apply(ctx, L.op, L.d, L.w));
+      hipLaunchKernelGGL((k_cheb_step<T>), dim3(nblk(n)), dim3(256), 0, st, L.w, L.r, L.dinv,
+                         L.d, L.x, rho_n * rho, 2.0 * rho_n / delta, n);
+      SEM_HIP(hipGetLastError());
+    }
     rho = rho_n;
   }
-  if (want_r) {
-    SEM_TRY(apply(ctx, L.op, L.d, L.w));
-    hipLaunchKernelGGL((k_sub<T>), dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
-    SEM_HIP(hipGetLastError());
+  if (want_r) SEM_TRY(residual_update<T>(ctx, L, L.d));
+  return SEM_OK;
+}
+
+// transfer launches; the geometry is checked against the kernels' limits
+template <typename T>
+int launch_prolong(hipStream_t st, const T* xc, const T* J, T* e, T* x, const Xfer& g) {
+  if (g.pc > kXferMaxP || g.pf > kXferMaxP || g.NCf % g.pf != 0) {
+    set_error("multigrid prolongation: unsupported transfer (%d -> %d points)", g.pc, g.pf);
+    return SEM_ERR_UNSUPPORTED;
   }
+  hipLaunchKernelGGL((k_prolong<T>), dim3(nblk(g.Lf), g.NCf / g.pf), dim3(256), 0, st, xc, J, e,
+                     x, g);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+template <typename T>
+int launch_restrict(hipStream_t st, const T* rf, const T* J, T* rc, const Xfer& g) {
+  const int fw = (g.hx ? 2 : 1) * g.pf, nfc = (g.hy ? 2 : 1) * g.pf;
+  if (fw > 256 || g.pc > kXferMaxP || g.NCc % g.pc != 0) {
+    set_error("multigrid restriction: unsupported transfer (%d -> %d points)", g.pf, g.pc);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  const int FR = (256 / fw) * fw, rows = (256 / fw) * g.pc;
+  const int nsx = g.hx ? 2 : 1;
+  const size_t lds =
+      sizeof(T) * ((size_t)FR * nfc + (size_t)rows * nfc + (size_t)nsx * g.pf * g.pc + (size_t)nfc * g.pc);
+  if (lds > 65536) {
+    set_error("multigrid restriction: %zu bytes of LDS for %d -> %d points", lds, g.pf, g.pc);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL((k_restrict<T>), dim3((g.Lf + FR - 1) / FR, g.NCc / g.pc), dim3(256), lds, st,
+                     rf, J, rc, g);
+  SEM_HIP(hipGetLastError());
   return SEM_OK;
 }
 
@@ -364,26 +490,20 @@ template <typename T>
 int vcycle(sem_ctx* ctx, sem_mg* mg, int l) {
   hipStream_t st = ctx->stream;
   sem_mg::Level& Lv = mg->lv[l];
-  const LvBuf<T> L = bufs<T>(Lv);
+  LvBuf<T> L = bufs<T>(Lv);
   const int last = (int)mg->lv.size() - 1;
   if (l == last) return cheby<T>(ctx, mg, Lv, L.b, 1, mg->prm.degree_coarse, false);
   SEM_TRY(cheby<T>(ctx, mg, Lv, L.b, 1, mg->prm.degree, true));
+  L = bufs<T>(Lv);   // the smoother may have swapped d and w
   sem_mg::Level& Cv = mg->lv[l + 1];
   const LvBuf<T> C = bufs<T>(Cv);
-  const long nc = (long)Cv.mesh->nl;
-  hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(nc)), dim3(256), 0, st, L.r, L.J, C.b, Lv.xf,
-                     nc);
+  SEM_TRY(launch_restrict<T>(st, L.r, L.J, C.b, Lv.xf));
   SEM_HIP(hipGetLastError());
   SEM_TRY(launch_gsmap_masked(st, Cv.mesh->map, Cv.mesh->layout(), Cv.op->desc.bc_bits, C.b,
                               C.b));
   SEM_TRY(vcycle<T>(ctx, mg, l + 1));
-  const long n = (long)Lv.mesh->nl;
-  hipLaunchKernelGGL((k_prolong<T>), dim3(nblk(n)), dim3(256), 0, st, C.x, L.J, L.e, L.x, Lv.xf,
-                     n);
-  SEM_HIP(hipGetLastError());
-  SEM_TRY(apply(ctx, L.op, L.e, L.w));
-  hipLaunchKernelGGL((k_sub<T>), dim3(nblk(n)), dim3(256), 0, st, L.r, L.w, n);
-  SEM_HIP(hipGetLastError());
+  SEM_TRY(launch_prolong<T>(st, C.x, L.J, L.e, L.x, Lv.xf));
+  SEM_TRY(residual_update<T>(ctx, L, L.e));
   return cheby<T>(ctx, mg, Lv, L.b, 0, mg->prm.degree, false);
 }
 
@@ -535,6 +655,7 @@ int build(sem_ctx* ctx, sem_mg* mg) {
     SEM_TRY(level_buffers(L, l > 0));
     SEM_TRY(sem_operator_jacobi(ctx, L.op, L.dinv));
     SEM_TRY(estimate_lmax(ctx, mg, L));   // uses the level's d, w, e only
+    L.fuse64 = mg->fuse && operator_epi_ok(L.op);
   }
   // the FP32 cycle: every level's operator rebuilt in FP32 and FP32 copies
   // of the diagonals and transfer rows (the eigenvalue bounds are the FP64
@@ -551,6 +672,7 @@ int build(sem_ctx* ctx, sem_mg* mg) {
       sem_operator* o32 = nullptr;
       SEM_TRY(sem_operator_create(ctx, L.mesh, &d, SEM_F32, &o32));
       L.op32 = o32;
+      L.fuse32 = mg->fuse && operator_epi_ok(o32);
       float** fb[] = {&L.fx, &L.fb, &L.fr, &L.fd, &L.fw, &L.fe, &L.fdinv};
       for (float** q : fb) SEM_TRY(dalloc(q, n));
       SEM_TRY(launch_convert_f64_f32(st, n, L.dinv, L.fdinv));
@@ -619,6 +741,8 @@ int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* par
   mg->device = ctx->device;
   mg->op = op;
   mg->f32 = p.precision == SEM_F32;
+  const char* fz = std::getenv("SEM_MG_FUSE");
+  mg->fuse = !(fz && std::atoi(fz) == 0);
   if (dalloc(&mg->part, kParts + 1) != SEM_OK) {
     mg_free(mg);
     return SEM_ERR_HIP;

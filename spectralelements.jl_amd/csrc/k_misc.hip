@@ -240,18 +240,38 @@ __device__ double block_reduce_max(double v, double* sh) {
   return r;
 }
 
+// Block partials of sum(a .* b .* mult): block q takes columns
+// [q cpb, (q + 1) cpb) whole (the column's multiplicity is uniform), its
+// threads the rows, four rows per pass with their loads issued together (one
+// block per CU keeps few waves resident), so there is no 64-bit division per
+// node and enough loads in flight.
 __global__ void __launch_bounds__(256)
 k_wdot_part(const double* __restrict__ a, const double* __restrict__ b, int P, int L, int NC,
-            int per_x, int per_y, long n, double* part) {
+            int per_x, int per_y, double* part) {
   __shared__ double sh[256];
-  double acc = 0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % L);
-    const int j = (int)(t / L);
-    acc += a[t] * b[t] * (mult1(i, P, L, per_x) * mult1(j, P, NC, per_y));
+  const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int j0 = blockIdx.x * cpb, j1 = min(NC, j0 + cpb);
+  const int bs = blockDim.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = j0; j < j1; ++j) {
+    const double mj = mult1(j, P, NC, per_y);
+    const long cj = (long)L * j;
+    for (int i0 = threadIdx.x; i0 < L; i0 += 4 * bs) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u * bs, L - 1);
+        av[u] = a[cj + i];
+        bv[u] = b[cj + i];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * bs;
+        if (i < L) acc[u] += av[u] * bv[u] * (mult1(i, P, L, per_x) * mj);
+      }
+    }
   }
-  const double s = block_reduce_sum(acc, sh);
+  const double s = block_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -776,9 +796,8 @@ int launch_abu(hipStream_t st, int m, int n, const double* u, const double* As, 
 
 int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* b, double* part,
                 int nparts, double* out) {
-  const long n = (long)g.L * g.NC;
   hipLaunchKernelGGL(k_wdot_part, dim3(nparts), dim3(256), 0, st, a, b, g.P, g.L, g.NC, g.per_x,
-                     g.per_y, n, part);
+                     g.per_y, part);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
   SEM_HIP(hipGetLastError());
   return SEM_OK;

@@ -134,6 +134,19 @@ int launch_geometry(hipStream_t st, sem_mesh* m);
 int launch_mult(hipStream_t st, const sem_mesh* m);
 int launch_mask_fill(hipStream_t st, const Layout& g, int bits, double* M);
 
+// Elementwise epilogue fused into an assembled apply (multigrid smoothing),
+// applied to each node copy where its assembled, masked value w = (A u)_i is
+// final: kind 1 (a Chebyshev step, u = d): r -= w; d' = c1 d + c2 dinv .* r;
+// x += d', with d' stored in y; kind 2: r -= w (y is then scratch). kind 0:
+// a plain apply.
+struct ApplyEpi {
+  int kind = 0;
+  void* r = nullptr;
+  void* x = nullptr;
+  const void* dinv = nullptr;
+  double c1 = 0.0, c2 = 0.0;
+};
+
 struct ApplyArgs {
   const void* u;
   void* y;
@@ -160,11 +173,17 @@ struct ApplyArgs {
   // for 16x16 blocks instead of the exact FP64/FP32 MFMA forms)
   int rows;
   int fma16;
+  ApplyEpi epi;
 };
 template <typename T>
 int launch_apply(hipStream_t st, const Layout& g, const ApplyArgs& a);
 // launch arguments of an operator apply y = op(u)
 ApplyArgs args_of(const sem_operator* op, const void* u, void* y);
+// the apply with a fused epilogue: SEM_ERR_UNSUPPORTED (nothing launched, no
+// error recorded) unless operator_epi_ok(op)
+bool operator_epi_ok(const sem_operator* op);
+int operator_apply_epi(sem_ctx* ctx, const sem_operator* op, const void* u, void* y,
+                       const ApplyEpi& e);
 
 // Device-resident PCG scalars (pcg.jl:23-55). t[k & 1] holds t = <r, h> of
 // iteration k; done/iters/converged/res are set once, by the first kernel of

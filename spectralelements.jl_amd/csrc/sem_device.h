@@ -158,6 +158,49 @@ __device__ __forceinline__ T mval(int i, int j, int L, int NC, int bc) {
   return mask_on(i, j, L, NC, bc) ? T(1) : T(0);
 }
 
+// Chebyshev-Jacobi direction update d' = c1 d + c2 dinv .* r: the smoother's
+// vector step and the fused apply epilogue evaluate this one expression with
+// an explicit fused multiply-add (left to the compiler, the contraction it
+// picks may differ between the two call sites), so the fused and unfused
+// smoothers give identical bits.
+template <typename T>
+__device__ __forceinline__ T cheb_dir(T c1, T d, T c2, T dinv, T r) {
+  const T t = c2 * (dinv * r);
+  if constexpr (sizeof(T) == 8) return __builtin_fma(c1, d, t);
+  else return __builtin_fmaf(c1, d, t);
+}
+
+// Fused apply epilogue (the kinds of ApplyEpi) for node copy t: its
+// operands are loaded first, together with the apply's other loads, and the
+// update is stored once the copy's assembled, masked value w is final: r -= w,
+// and for KIND 1 the direction update d' (stored in y; dold = the copy's
+// input value) and x += d'.
+template <typename T>
+struct EpiIn {
+  T r, dinv, x;
+};
+template <typename T, int KIND>
+__device__ __forceinline__ EpiIn<T> epi_load(long t, const T* r, const T* x, const T* dinv) {
+  EpiIn<T> v{};
+  v.r = r[t];
+  if constexpr (KIND == 1) {
+    v.dinv = dinv[t];
+    v.x = x[t];
+  }
+  return v;
+}
+template <typename T, int KIND>
+__device__ __forceinline__ void epi_store(long t, T w, T dold, const EpiIn<T>& v, T* r, T* x,
+                                          T c1, T c2, T* y) {
+  const T rv = v.r - w;
+  r[t] = rv;
+  if constexpr (KIND == 1) {
+    const T dv = cheb_dir(c1, dold, c2, v.dinv, rv);
+    y[t] = dv;
+    x[t] = v.x + dv;
+  }
+}
+
 // Seam geometry of a fused apply launch (see AxPlan): strips of CXP local
 // rows, segments of SP local columns.
 struct SeamGeom {

@@ -192,6 +192,25 @@ ApplyArgs args_of(const sem_operator* op, const void* u, void* y) {
   return a;
 }
 
+bool operator_epi_ok(const sem_operator* op) {
+  if (!op || op->generic || !op->desc.assemble) return false;
+  if (op->precision != SEM_F64 && op->precision != SEM_F32) return false;
+  const sem_mesh* m = op->mesh;
+  if (!(m->nr == 2 || m->nr == 4 || m->nr == 8)) return false;
+  return make_plan(m->layout(), true, false, op->ax_rows).RY == 2;
+}
+
+int operator_apply_epi(sem_ctx* ctx, const sem_operator* op, const void* u, void* y,
+                       const ApplyEpi& e) {
+  SEM_ARG(ctx && op && u && y, "NULL argument");
+  SEM_ARG(u != y, "fused apply: u and y must not alias");
+  if (!operator_epi_ok(op)) return SEM_ERR_UNSUPPORTED;
+  ApplyArgs a = args_of(op, u, y);
+  a.epi = e;
+  if (op->precision == SEM_F64) return launch_apply<double>(ctx->stream, op->mesh->layout(), a);
+  return launch_apply<float>(ctx->stream, op->mesh->layout(), a);
+}
+
 namespace {
 
 void free_mesh(sem_mesh* m) {

@@ -150,6 +150,43 @@ def test_pcg_mg_fp32_cycle_full_size(se, n, E):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("n,E,Ey,per,bc,prec", [
+    (8, 48, 40, (False, False), "DDDD", "f64"),
+    (8, 40, 36, (True, False), "NNDD", "f64"),
+    (8, 48, 40, (False, True), "DNDN", "f32"),
+    (16, 24, 24, (False, False), "DDDD", "f64"),
+])
+def test_fused_smoother_bits_equal_unfused(se, monkeypatch, n, E, Ey, per, bc, prec):
+    """The smoother's vector updates fused into the level applies (the
+    default, on the two-row apply levels p = 8, 4, 2) give exactly the bits
+    of the separate vector kernels (SEM_MG_FUSE=0 at creation): one V-cycle,
+    and a PCG solve with the same iterations and the same solution."""
+    import torch
+    from semhip import workloads as W
+    _log(f"fused smoother n={n} E={E}x{Ey} per={per} bc={bc} {prec}")
+    m = se.Mesh(n, n, E, Ey, per, se.deform_sinsin(0.1))
+    M = se.generateMask(bc, m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    b = W.helmholtz_rhs(se, m, M, seed=4)
+    tol = 1e-10 * float(b.abs().max())
+    Pf = se.MG(op, precision=prec)
+    monkeypatch.setenv("SEM_MG_FUSE", "0")
+    Pu = se.MG(op, precision=prec)
+    monkeypatch.delenv("SEM_MG_FUSE")
+    zf, zu = Pf(b), Pu(b)
+    assert torch.equal(zf, zu)
+    zf2, zu2 = Pf(b), Pu(b)   # the level buffers have traded roles in between
+    assert torch.equal(zf2, zf) and torch.equal(zu2, zu)
+    i_f, i_u = se.PCGInfo(), se.PCGInfo()
+    xf = se.pcg(b, op, opM=Pf, mult=m.mult, tol=tol, info=i_f)
+    xu = se.pcg(b, op, opM=Pu, mult=m.mult, tol=tol, info=i_u)
+    _log(f"  iterations {i_f.iters} / {i_u.iters}")
+    assert i_f.converged and i_f.iters == i_u.iters
+    assert torch.equal(xf, xu)
+    del Pf, Pu
+    torch.cuda.synchronize()
+
+
 def test_mg_rejects_field_coefficients(se):
     _log("mg rejects field nu")
     m = se.Mesh(5, 5, 4, 4)
