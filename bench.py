@@ -429,7 +429,9 @@ def probe(se, kind):
     'mfma': 11 assembled applies of each 16x16-block operator form (FP64,
     FP32, FP16, BF16, FP16 x3, BF16 x3) on 128x128 elements, N=15;
     'pcg': 100 Jacobi-PCG iterations at 256x256, N=7 (FP64) and at 128x128,
-    N=15 (FP32); 'cd': 60 cd2d steps (64x64, N=9) in the library's loop."""
+    N=15 (FP32); 'cd': 60 cd2d steps (64x64, N=9) in the library's loop;
+    'mg': the multigrid hierarchy set up and the 256x256, N=7 Helmholtz solve
+    to 1e-10 relative, twice."""
     import warnings
     import torch
     from semhip import workloads as W
@@ -472,6 +474,25 @@ def probe(se, kind):
         se.cd_objective(cdn, W.gaussian_blob(mV), 60)
         torch.cuda.synchronize()
         log("probe cd done")
+    elif kind == "mg":
+        m = se.Mesh(8, 8, 256, 256, (False, False), se.deform_sinsin(0.1))
+        M = se.generateMask("DDDD", m)
+        op = se.SEMOperator(1.0, 1.0, m, M)
+        b = W.helmholtz_rhs(se, m, M, seed=3)
+        tol = 1e-10 * float(b.abs().max())
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            P = se.MG(op)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            info = se.PCGInfo()
+            se.pcg(b, op, opM=P, mult=m.mult, tol=tol, info=info)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            log(f"probe mg {rep}: setup {1e3 * (t1 - t0):.1f} ms, solve {1e3 * (t2 - t1):.1f} ms, "
+                f"{info.iters} iterations")
+            del P
 
 
 def main():
@@ -486,7 +507,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cd", action="store_true")
     ap.add_argument("--cd-steps", type=int, default=2000)
-    ap.add_argument("--probe", choices=["mfma", "pcg", "cd"],
+    ap.add_argument("--probe", choices=["mfma", "pcg", "cd", "mg"],
                     help="run only a short counter-collection workload and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")

@@ -515,22 +515,32 @@ int estimate_lmax(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& L) {
   const Layout g = m->layout();
   double* v = L.d;
   double* w = L.w;
+  double* e = L.e;
   hipLaunchKernelGGL(k_hash_start, dim3(nblk(n)), dim3(256), 0, st, m->l2g, v, m->L, m->NC,
                      L.op->desc.bc_bits, n);
   SEM_HIP(hipGetLastError());
+  // v <- D^-1 A v; the Rayleigh quotient is scale-free, so the iterate is
+  // rescaled by its max norm only every kRenorm steps (|lambda| < 4 keeps the
+  // growth in between far from overflow), which saves the reduction and the
+  // division launches of the other steps
+  constexpr int kRenorm = 8;
   for (int it = 0; it < mg->prm.npow; ++it) {
     SEM_TRY(apply(ctx, L.op, v, w));
-    hipLaunchKernelGGL(k_mul2, dim3(nblk(n)), dim3(256), 0, st, L.dinv, w, L.e, n);
-    SEM_TRY(launch_norm_inf(st, n, L.e, mg->part, kParts, mg->part + kParts));
-    hipLaunchKernelGGL(k_div_scalar, dim3(nblk(n)), dim3(256), 0, st, L.e, mg->part + kParts, v,
-                       n);
+    hipLaunchKernelGGL(k_mul2, dim3(nblk(n)), dim3(256), 0, st, L.dinv, w, e, n);
+    if ((it + 1) % kRenorm == 0) {
+      SEM_TRY(launch_norm_inf(st, n, e, mg->part, kParts, mg->part + kParts));
+      hipLaunchKernelGGL(k_div_scalar, dim3(nblk(n)), dim3(256), 0, st, e, mg->part + kParts, v,
+                         n);
+    } else {
+      std::swap(v, e);
+    }
     SEM_HIP(hipGetLastError());
   }
   double num = 0.0, den = 0.0;
   SEM_TRY(apply(ctx, L.op, v, w));
   SEM_TRY(wdot_host(ctx, mg, g, v, w, &num));
-  hipLaunchKernelGGL(k_times_diag, dim3(nblk(n)), dim3(256), 0, st, v, L.dinv, L.e, n);
-  SEM_TRY(wdot_host(ctx, mg, g, v, L.e, &den));
+  hipLaunchKernelGGL(k_times_diag, dim3(nblk(n)), dim3(256), 0, st, v, L.dinv, e, n);
+  SEM_TRY(wdot_host(ctx, mg, g, v, e, &den));
   if (!(den > 0.0) || !(num > 0.0) || !std::isfinite(num / den)) {
     set_error("sem_mg_create: eigenvalue estimate failed on a level with p = %d, %d x %d "
               "elements (Rayleigh quotient %g / %g)", m->nr, m->Ex, m->Ey, num, den);
