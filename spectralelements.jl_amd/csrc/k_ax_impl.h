@@ -38,6 +38,16 @@
 // Tuning constants: threads per strip workgroup (one per local row, so a
 // strip is SEM_AX_THREADS / P elements wide) and the largest P * sizeof(T)
 // for which the geometric factors are staged one phase early.
+// Stores of the kernel's outputs as non-temporal (streaming) writes: the
+// apply's y (MODE 0 and 1; plain applies stream from HBM 0.7 us faster at
+// 256x256, N = 7, MALL-resident ones 1 us slower) and the PCG iteration's p,
+// x and w (MODE 2; off: its outputs are read again at once).
+#ifndef SEM_AX_NT_STORE
+#define SEM_AX_NT_STORE 1
+#endif
+#ifndef SEM_AX_NT_PCG
+#define SEM_AX_NT_PCG 0
+#endif
 #ifndef SEM_AX_THREADS
 #define SEM_AX_THREADS 256
 #endif
@@ -290,6 +300,11 @@ __global__ void __launch_bounds__(CX* P)
   };
 
   const int tid = threadIdx.x;
+  constexpr bool kNT = (MODE == 2) ? SEM_AX_NT_PCG : SEM_AX_NT_STORE;
+  auto st_out = [&](T* dst, T v) {
+    if constexpr (kNT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+  };
   // PCG: the iterate is advanced here, one iteration late
   constexpr bool kXLate = MODE == 2;
   T beta = T(0), alpha_prev = T(0);
@@ -428,7 +443,7 @@ __global__ void __launch_bounds__(CX* P)
         for (int b = 0; b < P; ++b) uv[b] = zv[b] + beta * po[b];
         if (act) {
 #pragma unroll
-          for (int b = 0; b < P; ++b) prm.x[base + L * b] = xo[b] + alpha_prev * po[b];
+          for (int b = 0; b < P; ++b) st_out(&prm.x[base + L * b], xo[b] + alpha_prev * po[b]);
         }
       } else {
 #pragma unroll
@@ -436,7 +451,7 @@ __global__ void __launch_bounds__(CX* P)
       }
       if (act) {
 #pragma unroll
-        for (int b = 0; b < P; ++b) prm.p[base + L * b] = uv[b];
+        for (int b = 0; b < P; ++b) st_out(&prm.p[base + L * b], uv[b]);
       }
     } else {
 #pragma unroll
@@ -694,7 +709,7 @@ __global__ void __launch_bounds__(CX* P)
     if (MODE == 0) {
       if (act) {
 #pragma unroll
-        for (int b = 0; b < P; ++b) prm.y[base + L * b] = h[b];
+        for (int b = 0; b < P; ++b) st_out(&prm.y[base + L * b], h[b]);
       }
       if constexpr (MF != 0) __syncthreads();  // s0, s1 were read in R3
       continue;
@@ -735,7 +750,7 @@ __global__ void __launch_bounds__(CX* P)
       // slot among this thread's preloaded epilogue operands)
       auto fin = [&](int idx, int j, T val, T uin, int eb) {
         const T wv = val * mval<T>(i, j, L, NC, bc);
-        if constexpr (EPI == 0) prm.y[idx] = wv;
+        if constexpr (EPI == 0) st_out(&prm.y[idx], wv);
         else epi_store<T, EPI>(idx, wv, uin, ein[eb], prm.er, prm.ex, prm.ec1, prm.ec2, prm.y);
       };
 #pragma unroll
@@ -747,18 +762,18 @@ __global__ void __launch_bounds__(CX* P)
           if (ysh) {
             if (ey > ey0) {
               if (xseam) {
-                prm.y[idx] = h[0];
-                if (RY == 1) prm.y[idx - L] = carry;
+                st_out(&prm.y[idx], h[0]);
+                if (RY == 1) st_out(&prm.y[idx - L], carry);
               } else {
                 const T s = carry + h[0];
                 fin(idx, j, s, uv[0], 0);
                 if (RY == 1) fin(idx - L, j - 1, s, carry_u, 0);
               }
             } else {
-              prm.y[idx] = h[0];  // y seam: finished by k_seams
+              st_out(&prm.y[idx], h[0]);  // y seam: finished by k_seams
             }
           } else {
-            if (xseam) prm.y[idx] = h[0];
+            if (xseam) st_out(&prm.y[idx], h[0]);
             else fin(idx, j, h[0], uv[0], 0);
           }
         } else if (b == P - 1) {
@@ -766,21 +781,21 @@ __global__ void __launch_bounds__(CX* P)
           if (ysh) {
             if (ey < ey1 - 1) {
               if constexpr (RY > 1) {
-                if (xseam) prm.y[idx] = h[P - 1];
+                if (xseam) st_out(&prm.y[idx], h[P - 1]);
                 else fin(idx, j, h[P - 1] + carry_hi, uv[P - 1], P - 1);
               } else {
                 carry = h[P - 1];
                 carry_u = uv[P - 1];
               }
             } else {
-              prm.y[idx] = h[P - 1];  // y seam
+              st_out(&prm.y[idx], h[P - 1]);  // y seam
             }
           } else {
-            if (xseam) prm.y[idx] = h[P - 1];
+            if (xseam) st_out(&prm.y[idx], h[P - 1]);
             else fin(idx, j, h[P - 1], uv[P - 1], P - 1);
           }
         } else {
-          if (xseam) prm.y[idx] = h[b];
+          if (xseam) st_out(&prm.y[idx], h[b]);
           else fin(idx, j, h[b], uv[b], b);
         }
       }
