@@ -48,6 +48,24 @@
 #ifndef SEM_AX_NT_PCG
 #define SEM_AX_NT_PCG 0
 #endif
+// unsummed seam copies (re-read by the seam pass) with ordinary stores
+// (42.7 vs 42.3 us with streaming ones: off)
+#ifndef SEM_AX_SEAM_CACHED
+#define SEM_AX_SEAM_CACHED 0
+#endif
+// the geometric factors and mass read with non-temporal loads (MODE 0 and 1;
+// MODE 2), and u (MODE 0 and 1). At 256x256, N = 7 from HBM: factors in
+// the apply 42.3 -> 38.9 us (MALL-resident: 36.7 -> 39.3 us); factors in the
+// PCG iteration 88.6 -> 89.4 us; u as well 40.7 us. Only the first is on.
+#ifndef SEM_AX_NT_LOAD_G
+#define SEM_AX_NT_LOAD_G 1
+#endif
+#ifndef SEM_AX_NT_LOAD_G_PCG
+#define SEM_AX_NT_LOAD_G_PCG 0
+#endif
+#ifndef SEM_AX_NT_LOAD_U
+#define SEM_AX_NT_LOAD_U 0
+#endif
 #ifndef SEM_AX_THREADS
 #define SEM_AX_THREADS 256
 #endif
@@ -284,18 +302,22 @@ __global__ void __launch_bounds__(CX* P)
   constexpr bool kHalfG = MF == 1 || MF == 2;
   auto ld_g11 = [&](int idx) -> T {
     if constexpr (kHalfG) return T((float)prm.h11[idx] * prm.gs[0]);
+    else if constexpr (MODE == 2 ? SEM_AX_NT_LOAD_G_PCG : SEM_AX_NT_LOAD_G) return __builtin_nontemporal_load(&prm.G11[idx]);
     else return prm.G11[idx];
   };
   auto ld_g12 = [&](int idx) -> T {
     if constexpr (kHalfG) return T((float)prm.h12[idx] * prm.gs[1]);
+    else if constexpr (MODE == 2 ? SEM_AX_NT_LOAD_G_PCG : SEM_AX_NT_LOAD_G) return __builtin_nontemporal_load(&prm.G12[idx]);
     else return prm.G12[idx];
   };
   auto ld_g22 = [&](int idx) -> T {
     if constexpr (kHalfG) return T((float)prm.h22[idx] * prm.gs[2]);
+    else if constexpr (MODE == 2 ? SEM_AX_NT_LOAD_G_PCG : SEM_AX_NT_LOAD_G) return __builtin_nontemporal_load(&prm.G22[idx]);
     else return prm.G22[idx];
   };
   auto ld_b = [&](int idx) -> T {
     if constexpr (kHalfG) return T((float)prm.hB[idx] * prm.gs[3]);
+    else if constexpr (MODE == 2 ? SEM_AX_NT_LOAD_G_PCG : SEM_AX_NT_LOAD_G) return __builtin_nontemporal_load(&prm.B[idx]);
     else return prm.B[idx];
   };
 
@@ -303,6 +325,10 @@ __global__ void __launch_bounds__(CX* P)
   constexpr bool kNT = (MODE == 2) ? SEM_AX_NT_PCG : SEM_AX_NT_STORE;
   auto st_out = [&](T* dst, T v) {
     if constexpr (kNT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+  };
+  auto st_seam = [&](T* dst, T v) {
+    if constexpr (kNT && !SEM_AX_SEAM_CACHED) __builtin_nontemporal_store(v, dst);
     else *dst = v;
   };
   // PCG: the iterate is advanced here, one iteration late
@@ -455,7 +481,10 @@ __global__ void __launch_bounds__(CX* P)
       }
     } else {
 #pragma unroll
-      for (int b = 0; b < P; ++b) uv[b] = prm.u[bc_ + L * b];
+      for (int b = 0; b < P; ++b) {
+        if constexpr (SEM_AX_NT_LOAD_U) uv[b] = __builtin_nontemporal_load(&prm.u[bc_ + L * b]);
+        else uv[b] = prm.u[bc_ + L * b];
+      }
       load_g();
     }
     if constexpr (MF == 0) {
@@ -589,7 +618,10 @@ __global__ void __launch_bounds__(CX* P)
     } else {
       const T* bsrc = (prm.has_mass && plain) ? prm.B : prm.G22;
 #pragma unroll
-      for (int b = 0; b < P; ++b) bv[b] = bsrc[bc_ + L * b];
+      for (int b = 0; b < P; ++b) {
+        if constexpr (MODE == 2 ? SEM_AX_NT_LOAD_G_PCG : SEM_AX_NT_LOAD_G) bv[b] = __builtin_nontemporal_load(&bsrc[bc_ + L * b]);
+        else bv[b] = bsrc[bc_ + L * b];
+      }
     }
     __syncthreads();
     if constexpr (MF != 0) {
@@ -762,18 +794,18 @@ __global__ void __launch_bounds__(CX* P)
           if (ysh) {
             if (ey > ey0) {
               if (xseam) {
-                st_out(&prm.y[idx], h[0]);
-                if (RY == 1) st_out(&prm.y[idx - L], carry);
+                st_seam(&prm.y[idx], h[0]);
+                if (RY == 1) st_seam(&prm.y[idx - L], carry);
               } else {
                 const T s = carry + h[0];
                 fin(idx, j, s, uv[0], 0);
                 if (RY == 1) fin(idx - L, j - 1, s, carry_u, 0);
               }
             } else {
-              st_out(&prm.y[idx], h[0]);  // y seam: finished by k_seams
+              st_seam(&prm.y[idx], h[0]);  // y seam: finished by k_seams
             }
           } else {
-            if (xseam) st_out(&prm.y[idx], h[0]);
+            if (xseam) st_seam(&prm.y[idx], h[0]);
             else fin(idx, j, h[0], uv[0], 0);
           }
         } else if (b == P - 1) {
@@ -781,21 +813,21 @@ __global__ void __launch_bounds__(CX* P)
           if (ysh) {
             if (ey < ey1 - 1) {
               if constexpr (RY > 1) {
-                if (xseam) st_out(&prm.y[idx], h[P - 1]);
+                if (xseam) st_seam(&prm.y[idx], h[P - 1]);
                 else fin(idx, j, h[P - 1] + carry_hi, uv[P - 1], P - 1);
               } else {
                 carry = h[P - 1];
                 carry_u = uv[P - 1];
               }
             } else {
-              st_out(&prm.y[idx], h[P - 1]);  // y seam
+              st_seam(&prm.y[idx], h[P - 1]);  // y seam
             }
           } else {
-            if (xseam) st_out(&prm.y[idx], h[P - 1]);
+            if (xseam) st_seam(&prm.y[idx], h[P - 1]);
             else fin(idx, j, h[P - 1], uv[P - 1], P - 1);
           }
         } else {
-          if (xseam) st_out(&prm.y[idx], h[b]);
+          if (xseam) st_seam(&prm.y[idx], h[b]);
           else fin(idx, j, h[b], uv[b], b);
         }
       }
