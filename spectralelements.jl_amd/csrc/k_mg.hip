@@ -17,6 +17,7 @@
 #include <new>
 #include <vector>
 
+#include "_gen/gll_dtables.h"
 #include "sem_device.h"
 #include "sem_l0.h"
 
@@ -203,6 +204,247 @@ __global__ void k_coarse_coords(const double* __restrict__ xf, const double* __r
   xc[t] = v;
 }
 
+// ---- the bilinear coarse levels in one workgroup ---------------------------
+// Levels of 2 x 2-node elements with at most kBlockNodes local nodes are
+// cycled by one 1024-thread workgroup, phase after phase with a barrier in
+// between, instead of one launch per vector pass (each of those is bound by
+// dispatch latency at these sizes). The element operator follows the fused
+// apply's P = 2 arithmetic; QQ' sums x-pairs, then y-pairs, as the fused
+// apply and its seam pass do; restriction, the coarse QQ' and prolongation
+// are the one-node-per-thread forms of the separate kernels.
+constexpr int kBlockThreads = 1024;
+constexpr long kBlockNodes = 1024;
+constexpr int kBlockMaxLevels = 12;
+constexpr int kBlockMaxDeg = 16;
+
+template <typename T>
+struct BlockLevel {
+  int Ex, Ey, L, NC, per_x, per_y, bc, has_mass;
+  int n;                         // local nodes
+  long ng;                       // global nodes (its QQ' map)
+  const int32_t* perm;
+  const int32_t* seg;
+  const T *G11, *G12, *G22, *B;
+  T nu, k;
+  T *x, *b, *r, *d, *w, *e;
+  const T* dinv;
+  const T* J;                    // transfer rows from the next level onto this one
+  Xfer xf;
+  int deg;                       // Chebyshev degree on this level
+  double theta;
+  double c1[kBlockMaxDeg], c2[kBlockMaxDeg];   // step i = 1 .. deg - 1
+};
+template <typename T>
+struct BlockCycle {
+  int nlev;
+  BlockLevel<T> lv[kBlockMaxLevels];
+};
+
+// element-local H u at every copy (the fused apply's P = 2 operations)
+template <typename T>
+__device__ void blk_local(const BlockLevel<T>& V, const T* u, T* h) {
+#define D2(ix) T(gll_d<2>(ix))
+  for (int t = threadIdx.x; t < V.n; t += blockDim.x) {
+    const int i = t % V.L, j = t / V.L;
+    const int a = i & 1, b = j & 1;
+    const int t0 = (i - a) + V.L * (j - b);
+    auto at = [&](int al, int be) { return t0 + al + V.L * be; };
+    T uu[2][2], g11[2][2], g12[2][2], g22[2][2];
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int be = 0; be < 2; ++be) {
+        uu[al][be] = u[at(al, be)];
+        g11[al][be] = V.G11[at(al, be)];
+        g12[al][be] = V.G12[at(al, be)];
+        g22[al][be] = V.G22[at(al, be)];
+      }
+    // us(al, be) = sum_k u(al, k) Ds(be + 2k); ur(al, be) = sum_k Dr(al + 2k) u(k, be)
+    T us[2][2], ur[2][2];
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int be = 0; be < 2; ++be) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) acc += uu[al][k] * D2(be + 2 * k);
+        us[al][be] = acc;
+        T acc2 = T(0);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) acc2 += D2(al + 2 * k) * uu[k][be];
+        ur[al][be] = acc2;
+      }
+    T wr[2][2], ws[2][2];
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int be = 0; be < 2; ++be) {
+        wr[al][be] = g11[al][be] * ur[al][be] + g12[al][be] * us[al][be];
+        ws[al][be] = g12[al][be] * ur[al][be] + g22[al][be] * us[al][be];
+      }
+    // this copy's row a and column b, selected (no indexing of register
+    // arrays by runtime values)
+    const T wsa[2] = {a ? ws[1][0] : ws[0][0], a ? ws[1][1] : ws[0][1]};
+    const T wrb[2] = {b ? wr[0][1] : wr[0][0], b ? wr[1][1] : wr[1][0]};
+    const T db[2] = {b ? D2(2) : D2(0), b ? D2(3) : D2(1)};   // Ds(k + 2b)
+    const T da[2] = {a ? D2(2) : D2(0), a ? D2(3) : D2(1)};   // Dr(k + 2a)
+    const T uab = a ? (b ? uu[1][1] : uu[1][0]) : (b ? uu[0][1] : uu[0][0]);
+    T p2 = T(0), p1 = T(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) p2 += wsa[k] * db[k];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) p1 += da[k] * wrb[k];
+    T hv;
+    if (V.has_mass) hv = V.nu * (p1 + p2) + V.k * (V.B[t] * uab);
+    else hv = V.nu * (p1 + p2);
+    h[t] = hv;
+  }
+#undef D2
+}
+
+// the assembled, masked value of every copy from the element-local values h,
+// fed to the epilogue (0: y = w; 1: Chebyshev step on r, d, x in place;
+// 2: r -= w)
+template <typename T, int KIND>
+__device__ void blk_final(const BlockLevel<T>& V, const T* h, T* y, double c1, double c2) {
+  for (int t = threadIdx.x; t < V.n; t += blockDim.x) {
+    const int i = t % V.L, j = t / V.L;
+    const int ex = i >> 1, a = i & 1, ey = j >> 1, b = j & 1;
+    const bool hx = a == 0 ? (ex > 0 || V.per_x) : (ex < V.Ex - 1 || V.per_x);
+    const bool hy = b == 0 ? (ey > 0 || V.per_y) : (ey < V.Ey - 1 || V.per_y);
+    const int ip = a == 0 ? (ex > 0 ? i - 1 : V.L - 1) : (ex < V.Ex - 1 ? i + 1 : 0);
+    const int jp = b == 0 ? (ey > 0 ? j - 1 : V.NC - 1) : (ey < V.Ey - 1 ? j + 1 : 0);
+    const int il = a == 0 ? ip : i, ir = a == 0 ? i : ip;   // left, right copies
+    const int jl = b == 0 ? jp : j, ju = b == 0 ? j : jp;   // lower, upper copies
+    auto xs = [&](int jj) {
+      const long cj = (long)V.L * jj;
+      return hx ? h[il + cj] + h[ir + cj] : h[i + cj];
+    };
+    const T s = hy ? xs(jl) + xs(ju) : xs(j);
+    const T wv = s * mval<T>(i, j, V.L, V.NC, V.bc);
+    if constexpr (KIND == 0) {
+      y[t] = wv;
+    } else {
+      const T rv = V.r[t] - wv;
+      V.r[t] = rv;
+      if constexpr (KIND == 1) {
+        const T dv = cheb_dir(T(c1), V.d[t], T(c2), V.dinv[t], rv);
+        V.d[t] = dv;
+        V.x[t] = V.x[t] + dv;
+      }
+    }
+  }
+}
+
+// Chebyshev smoothing of degree deg on level V (the separate kernels' steps;
+// from zero: x = 0, r = b), with the residual of the final x when want_r
+template <typename T>
+__device__ void blk_cheby(const BlockLevel<T>& V, int from_zero, int deg, bool want_r) {
+  for (int t = threadIdx.x; t < V.n; t += blockDim.x) {
+    T rv;
+    if (from_zero) {
+      rv = V.b[t];
+      V.r[t] = rv;
+    } else {
+      rv = V.r[t];
+    }
+    const T dv = V.dinv[t] * rv / T(V.theta);
+    V.d[t] = dv;
+    V.x[t] = from_zero ? dv : V.x[t] + dv;
+  }
+  __syncthreads();
+  for (int it = 1; it < deg; ++it) {
+    blk_local<T>(V, V.d, V.w);
+    __syncthreads();
+    blk_final<T, 1>(V, V.w, nullptr, V.c1[it], V.c2[it]);
+    __syncthreads();
+  }
+  if (want_r) {
+    blk_local<T>(V, V.d, V.w);
+    __syncthreads();
+    blk_final<T, 2>(V, V.w, nullptr, 0.0, 0.0);
+    __syncthreads();
+  }
+}
+
+// the V-cycle on levels lv[0] (the finest taken here) .. lv[nlev - 1]; lv[0].b
+// holds the right-hand side and lv[0].x receives the result
+template <typename T>
+__global__ void __launch_bounds__(kBlockThreads) k_block_cycle(const BlockCycle<T>* __restrict__ cc) {
+  const int nlev = cc->nlev;
+  for (int l = 0; l < nlev - 1; ++l) {
+    const BlockLevel<T>& F = cc->lv[l];
+    const BlockLevel<T>& C = cc->lv[l + 1];
+    blk_cheby<T>(F, 1, F.deg, true);
+    // restriction of r onto C.b (per coarse copy), then the coarse QQ', masked
+    const Xfer& g = F.xf;
+    const int nsx = g.hx ? 2 : 1, nsy = g.hy ? 2 : 1, blk = g.pf * g.pc;
+    for (int t = threadIdx.x; t < C.n; t += blockDim.x) {
+      const int i = t % g.Lc, j = t / g.Lc;
+      const int exc = i / g.pc, a = i - exc * g.pc;
+      const int eyc = j / g.pc, b = j - eyc * g.pc;
+      const T* Jy0 = F.J + nsx * blk;
+      T v = T(0);
+      for (int sy = 0; sy < nsy; ++sy) {
+        const int eyf = (eyc << g.hy) + sy;
+        for (int bf = 0; bf < g.pf; ++bf) {
+          const int jf = eyf * g.pf + bf;
+          const T wy = Jy0[sy * blk + bf * g.pc + b] * T(mult1(jf, g.pf, g.NCf, g.per_y));
+          T sacc = T(0);
+          for (int sx = 0; sx < nsx; ++sx) {
+            const int exf = (exc << g.hx) + sx;
+            for (int af = 0; af < g.pf; ++af) {
+              const int ifn = exf * g.pf + af;
+              sacc += F.J[sx * blk + af * g.pc + a] *
+                      (T(mult1(ifn, g.pf, g.Lf, g.per_x)) * F.r[ifn + (long)g.Lf * jf]);
+            }
+          }
+          v += wy * sacc;
+        }
+      }
+      C.b[t] = v;
+    }
+    __syncthreads();
+    for (long q = threadIdx.x; q < C.ng; q += blockDim.x)
+      gsmap_node(C.perm, C.seg, C.b, C.b, q, C.bc, C.L, C.NC);
+    __syncthreads();
+  }
+  {
+    const BlockLevel<T>& V = cc->lv[nlev - 1];
+    blk_cheby<T>(V, 1, V.deg, false);
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const BlockLevel<T>& F = cc->lv[l];
+    const BlockLevel<T>& C = cc->lv[l + 1];
+    const Xfer& g = F.xf;
+    const int blk = g.pf * g.pc;
+    for (int t = threadIdx.x; t < F.n; t += blockDim.x) {
+      const int i = t % g.Lf, j = t / g.Lf;
+      const int exf = i / g.pf, a = i - exf * g.pf;
+      const int eyf = j / g.pf, b = j - eyf * g.pf;
+      const int sx = exf & g.hx, sy = eyf & g.hy;
+      const int exc = exf >> g.hx, eyc = eyf >> g.hy;
+      const T* jx = F.J + sx * blk + a * g.pc;
+      const T* jy = F.J + (g.hx ? 2 : 1) * blk + sy * blk + b * g.pc;
+      const long c0 = (long)exc * g.pc + (long)g.Lc * ((long)eyc * g.pc);
+      T v = T(0);
+      for (int bb = 0; bb < g.pc; ++bb) {
+        T sacc = T(0);
+        for (int aa = 0; aa < g.pc; ++aa) sacc += jx[aa] * C.x[c0 + aa + (long)g.Lc * bb];
+        v += jy[bb] * sacc;
+      }
+      F.e[t] = v;
+      F.x[t] = F.x[t] + v;
+    }
+    __syncthreads();
+    blk_local<T>(F, F.e, F.w);
+    __syncthreads();
+    blk_final<T, 2>(F, F.w, nullptr, 0.0, 0.0);
+    __syncthreads();
+    blk_cheby<T>(F, 0, F.deg, false);
+  }
+}
+
 // hashed start vector of the power iteration: M .* (h(g) / 2^32 - 1/2),
 // h(g) = (g * 2654435761 + 12345) mod 2^32 of the global node id
 __global__ void k_hash_start(const int32_t* __restrict__ l2g, double* __restrict__ v, int L,
@@ -296,6 +538,13 @@ struct sem_mg {
   // smoother updates fused into the level applies (SEM_MG_FUSE=0 at
   // creation turns it off: a test hook)
   bool fuse = true;
+  // the bilinear coarse levels lc .. last cycled by one workgroup (lc < 0:
+  // none; SEM_MG_BLOCK=0 at creation turns it off), with their level
+  // descriptors in device memory per cycle precision
+  int lc = -1;
+  bool block = true;
+  void* bdesc64 = nullptr;
+  void* bdesc32 = nullptr;
   std::vector<Level> lv;
   // PCG vectors on the fine level (pq: the previous residual of the flexible
   // recurrence), reduction partials and the scalar slot
@@ -328,6 +577,8 @@ void mg_free(sem_mg* mg) {
   double* bufs[] = {mg->pr, mg->pp, mg->pw, mg->pz, mg->pq, mg->part};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
+  if (mg->bdesc64) (void)hipFree(mg->bdesc64);
+  if (mg->bdesc32) (void)hipFree(mg->bdesc32);
   delete mg;
 }
 
@@ -489,6 +740,13 @@ int launch_restrict(hipStream_t st, const T* rf, const T* J, T* rc, const Xfer& 
 template <typename T>
 int vcycle(sem_ctx* ctx, sem_mg* mg, int l) {
   hipStream_t st = ctx->stream;
+  if (l == mg->lc) {
+    const void* desc = sizeof(T) == 8 ? mg->bdesc64 : mg->bdesc32;
+    hipLaunchKernelGGL((k_block_cycle<T>), dim3(1), dim3(kBlockThreads), 0, st,
+                       static_cast<const BlockCycle<T>*>(desc));
+    SEM_HIP(hipGetLastError());
+    return SEM_OK;
+  }
   sem_mg::Level& Lv = mg->lv[l];
   LvBuf<T> L = bufs<T>(Lv);
   const int last = (int)mg->lv.size() - 1;
@@ -562,6 +820,93 @@ int level_buffers(sem_mg::Level& L, bool own_xb) {
     SEM_TRY(dalloc(&L.b, n));
   }
   return SEM_OK;
+}
+
+// the first level of the block cycle: the smallest l >= 1 from which every
+// level is bilinear with at most kBlockNodes local nodes and every transfer
+// a 2 x 2 element merge (-1: none)
+int block_start(const sem_mg* mg) {
+  const int nl = (int)mg->lv.size();
+  if (!mg->block || nl > kBlockMaxLevels + 1) return -1;
+  if (mg->prm.degree > kBlockMaxDeg || mg->prm.degree_coarse > kBlockMaxDeg) return -1;
+  int lc = -1;
+  for (int l = nl - 1; l >= 1; --l) {
+    const sem_mg::Level& L = mg->lv[l];
+    const sem_mesh* m = L.mesh;
+    bool ok = m->nr == 2 && m->ns == 2 && m->nl <= kBlockNodes;
+    if (l < nl - 1) ok = ok && L.h == 1 && L.xf.pf == 2 && L.xf.pc == 2;
+    if (!ok) break;
+    lc = l;
+  }
+  if (lc >= 0 && nl - lc > kBlockMaxLevels) lc = -1;
+  return lc;
+}
+
+template <typename T>
+int block_desc(sem_mg* mg, void** out) {
+  const int nl = (int)mg->lv.size();
+  const int lc = mg->lc;
+  BlockCycle<T>* h = new (std::nothrow) BlockCycle<T>();
+  if (!h) {
+    set_error("multigrid: out of host memory");
+    return SEM_ERR_HIP;
+  }
+  h->nlev = nl - lc;
+  for (int q = 0; q < h->nlev; ++q) {
+    sem_mg::Level& L = mg->lv[lc + q];
+    const LvBuf<T> B = bufs<T>(L);
+    const sem_operator* op = B.op;
+    const sem_mesh* m = L.mesh;
+    BlockLevel<T>& V = h->lv[q];
+    V.Ex = m->Ex;
+    V.Ey = m->Ey;
+    V.L = m->L;
+    V.NC = m->NC;
+    V.per_x = m->per_x;
+    V.per_y = m->per_y;
+    V.bc = op->desc.bc_bits;
+    V.has_mass = op->has_mass ? 1 : 0;
+    V.n = (int)m->nl;
+    V.ng = (long)m->map->n_global;
+    V.perm = m->map->perm;
+    V.seg = m->map->seg;
+    V.G11 = static_cast<const T*>(op->G11);
+    V.G12 = static_cast<const T*>(op->G12);
+    V.G22 = static_cast<const T*>(op->G22);
+    V.B = static_cast<const T*>(op->B);
+    V.nu = T(op->desc.nu);
+    V.k = T(op->desc.k);
+    V.x = B.x;
+    V.b = B.b;
+    V.r = B.r;
+    V.d = B.d;
+    V.w = B.w;
+    V.e = B.e;
+    V.dinv = B.dinv;
+    V.J = B.J;
+    V.xf = L.xf;
+    const bool last = lc + q == nl - 1;
+    V.deg = last ? mg->prm.degree_coarse : mg->prm.degree;
+    // the coefficients cheby() forms on the host, step by step
+    const double lmax = L.lmax, lmin = L.lmax / mg->prm.ratio;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    V.theta = theta;
+    for (int i = 1; i < V.deg; ++i) {
+      const double rho_n = 1.0 / (2.0 * sigma - rho);
+      V.c1[i] = rho_n * rho;
+      V.c2[i] = 2.0 * rho_n / delta;
+      rho = rho_n;
+    }
+  }
+  int s = dalloc(reinterpret_cast<unsigned char**>(out), (long)sizeof(BlockCycle<T>));
+  if (s == SEM_OK && hipMemcpy(*out, h, sizeof(BlockCycle<T>), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("multigrid: copying the block-cycle levels failed");
+    s = SEM_ERR_HIP;
+  }
+  delete h;
+  return s;
 }
 
 int build(sem_ctx* ctx, sem_mg* mg) {
@@ -698,6 +1043,11 @@ int build(sem_ctx* ctx, sem_mg* mg) {
   SEM_TRY(dalloc(&mg->pw, n0));
   SEM_TRY(dalloc(&mg->pz, n0));
   if (mg->f32) SEM_TRY(dalloc(&mg->pq, n0));
+  mg->lc = block_start(mg);
+  if (mg->lc >= 1) {
+    SEM_TRY(block_desc<double>(mg, &mg->bdesc64));
+    if (mg->f32) SEM_TRY(block_desc<float>(mg, &mg->bdesc32));
+  }
   SEM_HIP(hipStreamSynchronize(st));
   return SEM_OK;
 }
@@ -753,6 +1103,8 @@ int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* par
   mg->f32 = p.precision == SEM_F32;
   const char* fz = std::getenv("SEM_MG_FUSE");
   mg->fuse = !(fz && std::atoi(fz) == 0);
+  const char* fb = std::getenv("SEM_MG_BLOCK");
+  mg->block = !(fb && std::atoi(fb) == 0);
   if (dalloc(&mg->part, kParts + 1) != SEM_OK) {
     mg_free(mg);
     return SEM_ERR_HIP;

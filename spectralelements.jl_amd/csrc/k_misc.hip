@@ -2,6 +2,7 @@
 // jac.jl:24-40), mult, masks, the map-based gather-scatter, ABu, reductions,
 // grad / grad' and the adjoint dJ/dnu kernel, dealiased advection.
 #include "sem_common.h"
+#include "sem_device.h"
 
 namespace sem {
 namespace {
@@ -72,14 +73,6 @@ __global__ void k_geometry(const double* __restrict__ x, const double* __restric
   G22[t] = vB * (vsx * vsx + vsy * vsy);
 }
 
-__device__ __forceinline__ double mult1(int i, int n, int L, int per) {
-  // 1/(multiplicity along one direction) of local row i
-  const int a = i % n;
-  if (a == 0 && (i > 0 || per)) return 0.5;
-  if (a == n - 1 && (i < L - 1 || per)) return 0.5;
-  return 1.0;
-}
-
 __global__ void k_mult(double* __restrict__ mult, int nr, int ns, int L, int NC, int per_x,
                        int per_y, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,13 +83,6 @@ __global__ void k_mult(double* __restrict__ mult, int nr, int ns, int L, int NC,
   mult[t] = mult1(i, nr, L, per_x) * mult1(j, ns, NC, per_y);
 }
 
-__device__ __forceinline__ bool mask_on(int i, int j, int L, int NC, int bc) {
-  if ((bc & SEM_BC_XMIN) && i == 0) return false;
-  if ((bc & SEM_BC_XMAX) && i == L - 1) return false;
-  if ((bc & SEM_BC_YMIN) && j == 0) return false;
-  if ((bc & SEM_BC_YMAX) && j == NC - 1) return false;
-  return true;
-}
 
 __global__ void k_mask_fill(double* __restrict__ M, int L, int NC, int bc, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -132,32 +118,7 @@ __global__ void k_gsmap(const int32_t* __restrict__ perm, const int32_t* __restr
                         const T* u, T* y, long ng, int bc, int L, int NC) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
-  const int s0 = seg[g], s1 = seg[g + 1];
-  const int n = s1 - s0;
-  T s;
-  if (n == 1) {
-    s = u[perm[s0]];
-  } else if (n == 2) {
-    s = u[perm[s0]] + u[perm[s0 + 1]];
-  } else if (n == 4) {
-    s = (u[perm[s0]] + u[perm[s0 + 1]]) + (u[perm[s0 + 2]] + u[perm[s0 + 3]]);
-  } else {
-    // general segment: pairwise sums of consecutive pairs, left to right
-    s = T(0);
-    int k = s0;
-    bool first = true;
-    for (; k + 1 < s1; k += 2) {
-      const T p = u[perm[k]] + u[perm[k + 1]];
-      s = first ? p : s + p;
-      first = false;
-    }
-    if (k < s1) s = first ? u[perm[k]] : s + u[perm[k]];
-  }
-  if (bc >= 0) {
-    const int l = perm[s0];
-    s = s * (mask_on(l % L, l / L, L, NC, bc) ? T(1) : T(0));
-  }
-  for (int k = s0; k < s1; ++k) y[perm[k]] = s;
+  gsmap_node(perm, seg, u, y, g, bc, L, NC);
 }
 
 __global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y, long n) {

@@ -158,6 +158,42 @@ __device__ __forceinline__ T mval(int i, int j, int L, int NC, int bc) {
   return mask_on(i, j, L, NC, bc) ? T(1) : T(0);
 }
 
+// QQ' of one global node g through the sorted map (perm: local copies by
+// global id, seg: offsets): the copies' sum in a fixed order (pairs of
+// consecutive copies first), masked by the first copy's mask when bc >= 0,
+// written to every copy. u may alias y (all copies are read first).
+template <typename T>
+__device__ __forceinline__ void gsmap_node(const int32_t* __restrict__ perm,
+                                           const int32_t* __restrict__ seg, const T* u, T* y,
+                                           long g, int bc, int L, int NC) {
+  const int s0 = seg[g], s1 = seg[g + 1];
+  const int n = s1 - s0;
+  T s;
+  if (n == 1) {
+    s = u[perm[s0]];
+  } else if (n == 2) {
+    s = u[perm[s0]] + u[perm[s0 + 1]];
+  } else if (n == 4) {
+    s = (u[perm[s0]] + u[perm[s0 + 1]]) + (u[perm[s0 + 2]] + u[perm[s0 + 3]]);
+  } else {
+    // general segment: pairwise sums of consecutive pairs, left to right
+    s = T(0);
+    int k = s0;
+    bool first = true;
+    for (; k + 1 < s1; k += 2) {
+      const T p = u[perm[k]] + u[perm[k + 1]];
+      s = first ? p : s + p;
+      first = false;
+    }
+    if (k < s1) s = first ? u[perm[k]] : s + u[perm[k]];
+  }
+  if (bc >= 0) {
+    const int l = perm[s0];
+    s = s * (mask_on(l % L, l / L, L, NC, bc) ? T(1) : T(0));
+  }
+  for (int k = s0; k < s1; ++k) y[perm[k]] = s;
+}
+
 // Chebyshev-Jacobi direction update d' = c1 d + c2 dinv .* r: the smoother's
 // vector step and the fused apply epilogue evaluate this one expression with
 // an explicit fused multiply-add (left to the compiler, the contraction it

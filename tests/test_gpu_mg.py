@@ -187,6 +187,46 @@ def test_fused_smoother_bits_equal_unfused(se, monkeypatch, n, E, Ey, per, bc, p
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("n,E,Ey,per,bc,prec", [
+    (8, 64, 64, (False, False), "DDDD", "f64"),
+    (8, 48, 40, (True, False), "NNDD", "f64"),
+    (4, 32, 64, (False, True), "DNDN", "f64"),
+    (8, 64, 64, (False, False), "DDDD", "f32"),
+])
+def test_block_coarse_cycle_matches_launch_per_pass(se, monkeypatch, n, E, Ey, per, bc, prec):
+    """The bilinear coarse levels cycled by one workgroup (the default) against
+    one launch per pass (SEM_MG_BLOCK=0 at creation): the same operations in
+    the same order except for the compiler's choice of fused multiply-adds in
+    the element operator, so one V-cycle agrees to 1e-12 (FP64) / 1e-5 (FP32)
+    relative, and the PCG solves take the same iterations to the same
+    solution within 1e-10."""
+    import torch
+    from semhip import workloads as W
+    _log(f"block coarse cycle n={n} E={E}x{Ey} per={per} bc={bc} {prec}")
+    m = se.Mesh(n, n, E, Ey, per, se.deform_sinsin(0.1))
+    M = se.generateMask(bc, m)
+    op = se.SEMOperator(1.0, 1.0, m, M)
+    b = W.helmholtz_rhs(se, m, M, seed=6)
+    tol = 1e-10 * float(b.abs().max())
+    Pb = se.MG(op, precision=prec)
+    monkeypatch.setenv("SEM_MG_BLOCK", "0")
+    Pl = se.MG(op, precision=prec)
+    monkeypatch.delenv("SEM_MG_BLOCK")
+    zb, zl = Pb(b), Pl(b)
+    rel = float((zb - zl).abs().max()) / float(zl.abs().max())
+    _log(f"  V-cycle relative difference {rel:.2e}, bitwise {torch.equal(zb, zl)}")
+    assert rel <= (1e-12 if prec == "f64" else 1e-5)
+    ib, il = se.PCGInfo(), se.PCGInfo()
+    xb = se.pcg(b, op, opM=Pb, mult=m.mult, tol=tol, info=ib)
+    xl = se.pcg(b, op, opM=Pl, mult=m.mult, tol=tol, info=il)
+    _log(f"  iterations {ib.iters} / {il.iters}")
+    assert ib.converged and abs(ib.iters - il.iters) <= 1
+    assert float((xb - xl).abs().max()) <= 1e-10 * float(xl.abs().max())
+    assert float((b - op(xb)).abs().max()) <= 1.05 * tol
+    del Pb, Pl
+    torch.cuda.synchronize()
+
+
 def test_mg_rejects_field_coefficients(se):
     _log("mg rejects field nu")
     m = se.Mesh(5, 5, 4, 4)
