@@ -48,10 +48,16 @@
 #ifndef SEM_AX_NT_PCG
 #define SEM_AX_NT_PCG 0
 #endif
-// unsummed seam copies (re-read by the seam pass) with ordinary stores
-// (42.7 vs 42.3 us with streaming ones: off)
+// the seam pass's loads and stores of y as non-temporal ones (plain apply;
+// off: 40.0 vs 38.7 us at 256x256, N = 7)
+#ifndef SEM_SEAMS_NT
+#define SEM_SEAMS_NT 0
+#endif
+// unsummed seam copies (re-read by the seam pass) with ordinary stores: with
+// the factors' non-temporal loads 38.65 vs 38.7 us at 256x256, N = 7 (42.7 vs
+// 42.3 us before them)
 #ifndef SEM_AX_SEAM_CACHED
-#define SEM_AX_SEAM_CACHED 0
+#define SEM_AX_SEAM_CACHED 1
 #endif
 // the geometric factors and mass read with non-temporal loads (MODE 0 and 1;
 // MODE 2), and u (MODE 0 and 1). At 256x256, N = 7 from HBM: factors in
@@ -874,9 +880,16 @@ k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int
     }
     return c;
   };
+  auto ly = [&](long t) -> T {
+    if constexpr (SEM_SEAMS_NT && EPI == 0) return __builtin_nontemporal_load(&y[t]);
+    else return y[t];
+  };
   auto fin = [&](int i, int j, const Copy& c, T s) {
     const T wv = s * mval<T>(i, j, L, NC, bc);
-    if constexpr (EPI == 0) y[c.t] = wv;
+    if constexpr (EPI == 0) {
+      if constexpr (SEM_SEAMS_NT) __builtin_nontemporal_store(wv, &y[c.t]);
+      else y[c.t] = wv;
+    }
     else epi_store<T, EPI>(c.t, wv, c.u, c.e, er, ex, ec1, ec2, y);
   };
   if ((int)blockIdx.y < nys) {
@@ -896,7 +909,7 @@ k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int
                         (per_x && (i == 0 || i == L - 1));
     if (xs_row) return;   // summed with the four-copy rule by the x-seam items
     const Copy cL = pre(i, jL), cR = pre(i, jR);
-    const T sm = y[cL.t] + y[cR.t];
+    const T sm = ly(cL.t) + ly(cR.t);
     fin(i, jL, cL, sm);
     fin(i, jR, cR, sm);
   } else {
@@ -916,8 +929,8 @@ k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int
     if (b == P - 1 && (j < NC - 1 || per_y)) {
       const int jU = (j + 1 == NC) ? 0 : j + 1;
       const Copy c0 = pre(iL, j), c1 = pre(iR, j), c2 = pre(iL, jU), c3 = pre(iR, jU);
-      const T sl = y[c0.t] + y[c1.t];
-      const T su = y[c2.t] + y[c3.t];
+      const T sl = ly(c0.t) + ly(c1.t);
+      const T su = ly(c2.t) + ly(c3.t);
       const T sm = sl + su;
       fin(iL, j, c0, sm);
       fin(iR, j, c1, sm);
@@ -925,7 +938,7 @@ k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int
       fin(iR, jU, c3, sm);
     } else {
       const Copy c0 = pre(iL, j), c1 = pre(iR, j);
-      const T sm = y[c0.t] + y[c1.t];
+      const T sm = ly(c0.t) + ly(c1.t);
       fin(iL, j, c0, sm);
       fin(iR, j, c1, sm);
     }
