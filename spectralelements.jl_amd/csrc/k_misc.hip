@@ -652,9 +652,64 @@ int launch_geometry(hipStream_t st, sem_mesh* m) {
   return SEM_OK;
 }
 
+// ---- structured numbering on the device (semq.jl:6-25) -------------------
+// Per direction (small host tables): the global index of each local row
+// (gof), and for each global index its copies (c0, and c1 or -1 for a node
+// shared by two elements or the periodic wrap) and the exclusive prefix of the
+// copy counts (pre). Global node g = gx + nxg gy then owns the local copies
+// {ic + L jc} in ascending order, starting at seg[g] = prey[gy] L + cy[gy]
+// prex[gx]: the map the host builds by a counting sort of l2g, without the
+// host pass over every local node.
+struct AxisMap {
+  const int32_t* gof;
+  const int32_t* c0;
+  const int32_t* c1;
+  const int32_t* pre;
+};
+
+__global__ void k_l2g(int32_t* __restrict__ l2g, AxisMap ax, AxisMap ay, int L, int NC, long nxg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  for (int j = blockIdx.y; j < NC; j += gridDim.y)
+    l2g[i + (long)L * j] = (int32_t)(ax.gof[i] + nxg * ay.gof[j]);
+}
+
+__global__ void k_gsmap_build(int32_t* __restrict__ perm, int32_t* __restrict__ seg, AxisMap ax,
+                              AxisMap ay, int L, int nxg, int nyg, long ng, long nl) {
+  const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gx >= nxg) return;
+  for (int gy = blockIdx.y; gy < nyg; gy += gridDim.y) {
+  const long g = gx + (long)nxg * gy;
+  const int cx = ax.c1[gx] >= 0 ? 2 : 1, cy = ay.c1[gy] >= 0 ? 2 : 1;
+  const long s0 = (long)ay.pre[gy] * L + (long)cy * ax.pre[gx];
+  seg[g] = (int32_t)s0;
+  if (g == ng - 1) seg[ng] = (int32_t)nl;
+  long k = s0;
+  for (int q = 0; q < cy; ++q) {
+    const long jc = q == 0 ? ay.c0[gy] : ay.c1[gy];
+    for (int p = 0; p < cx; ++p) {
+      const int ic = p == 0 ? ax.c0[gx] : ax.c1[gx];
+      perm[k++] = (int32_t)(ic + L * jc);
+    }
+  }
+  }
+}
+
 int launch_mult(hipStream_t st, const sem_mesh* m) {
   hipLaunchKernelGGL(k_mult, dim3(nblk(m->nl)), dim3(256), 0, st, m->f[SEM_MULT], m->nr, m->ns,
                      m->L, m->NC, m->per_x, m->per_y, (long)m->nl);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
+int launch_numbering(hipStream_t st, const sem_mesh* m, const int32_t* const tabs[8], int nxg,
+                     int nyg, int32_t* l2g, int32_t* perm, int32_t* seg) {
+  const AxisMap ax{tabs[0], tabs[1], tabs[2], tabs[3]};
+  const AxisMap ay{tabs[4], tabs[5], tabs[6], tabs[7]};
+  hipLaunchKernelGGL(k_l2g, dim3(nblk(m->L), std::min(m->NC, 32768)), dim3(256), 0, st, l2g, ax,
+                     ay, m->L, m->NC, (long)nxg);
+  hipLaunchKernelGGL(k_gsmap_build, dim3(nblk(nxg), std::min(nyg, 32768)), dim3(256), 0, st, perm,
+                     seg, ax, ay, m->L, nxg, nyg, (long)nxg * nyg, (long)m->nl);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
 }

@@ -132,6 +132,11 @@ int launch_coords(hipStream_t st, const sem_mesh* m, const double* xe, const dou
                   int deform, double amp);
 int launch_geometry(hipStream_t st, sem_mesh* m);
 int launch_mult(hipStream_t st, const sem_mesh* m);
+// l2g and the QQ' map of a structured mesh from per-direction tables (x:
+// global index per local row, first copy, second copy or -1, prefix of copy
+// counts; then the same for y)
+int launch_numbering(hipStream_t st, const sem_mesh* m, const int32_t* const tabs[8], int nxg,
+                     int nyg, int32_t* l2g, int32_t* perm, int32_t* seg);
 int launch_mask_fill(hipStream_t st, const Layout& g, int bits, double* M);
 
 // Elementwise epilogue fused into an assembled apply (multigrid smoothing),

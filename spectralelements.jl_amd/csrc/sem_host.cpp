@@ -397,18 +397,63 @@ int sem_mesh_create(sem_ctx* ctx, int nr, int ns, int Ex, int Ey, int periodic_x
   hipFree(dye);
   if (s != SEM_OK) return fail(s);
   // local-to-global numbering (semq.jl:6-25 in both directions, mesh.jl:87-89)
-  std::vector<int32_t> l2g(m->nl);
-  for (int64_t j = 0; j < NC; ++j) {
-    int64_t gy = (j / ns) * (ns - 1) + (j % ns);
-    if (m->per_y) gy %= nyg;
-    for (int64_t i = 0; i < L; ++i) {
-      int64_t gx = (i / nr) * (nr - 1) + (i % nr);
-      if (m->per_x) gx %= nxg;
-      l2g[i + L * j] = (int32_t)(gx + nxg * gy);
+  // and the QQ' map, built on the device from per-direction tables: per
+  // local row its global index, per global index its one or two copies and
+  // the prefix of the copy counts
+  std::vector<int32_t> tab;
+  int64_t off[9];
+  auto axis = [&](int64_t Lr, int n, int per, int64_t ngx) -> bool {
+    std::vector<int32_t> gof(Lr), c0(ngx, -1), c1(ngx, -1), pre(ngx);
+    for (int64_t i = 0; i < Lr; ++i) {
+      int64_t g = (i / n) * (n - 1) + (i % n);
+      if (per) g %= ngx;
+      gof[i] = (int32_t)g;
+      if (c0[g] < 0) c0[g] = (int32_t)i;
+      else if (c1[g] < 0) c1[g] = (int32_t)i;
+      else return false;
     }
+    int32_t acc = 0;
+    for (int64_t g = 0; g < ngx; ++g) {
+      pre[g] = acc;
+      acc += c1[g] >= 0 ? 2 : 1;
+    }
+    for (auto* v : {&gof, &c0, &c1, &pre}) tab.insert(tab.end(), v->begin(), v->end());
+    return true;
+  };
+  if (!axis(L, nr, m->per_x, nxg) || !axis(NC, ns, m->per_y, nyg)) {
+    set_error("sem_mesh_create: a global index with more than two copies along one direction");
+    return fail(SEM_ERR_ARG);
   }
-  if ((s = upload(st, &m->l2g, l2g.data(), l2g.size())) != SEM_OK) return fail(s);
-  if ((s = build_gsmap(ctx, m->nl, l2g.data(), m->ng, &m->map)) != SEM_OK) return fail(s);
+  {
+    const int64_t sz[8] = {L, nxg, nxg, nxg, NC, nyg, nyg, nyg};
+    off[0] = 0;
+    for (int q = 0; q < 8; ++q) off[q + 1] = off[q] + sz[q];
+  }
+  int32_t* dtab = nullptr;
+  if ((s = upload(st, &dtab, tab.data(), tab.size())) != SEM_OK) return fail(s);
+  const int32_t* tabs[8];
+  for (int q = 0; q < 8; ++q) tabs[q] = dtab + off[q];
+  sem_gsmap* map = new (std::nothrow) sem_gsmap();
+  if (!map) {
+    hipFree(dtab);
+    set_error("out of host memory");
+    return fail(SEM_ERR_ARG);
+  }
+  map->n_local = m->nl;
+  map->n_global = m->ng;
+  map->device = ctx->device;
+  m->map = map;
+  if (hipMalloc(&m->l2g, sizeof(int32_t) * m->nl) != hipSuccess ||
+      hipMalloc(&map->perm, sizeof(int32_t) * m->nl) != hipSuccess ||
+      hipMalloc(&map->seg, sizeof(int32_t) * (m->ng + 1)) != hipSuccess) {
+    hipFree(dtab);
+    set_error("sem_mesh_create: hipMalloc of the numbering failed");
+    return fail(SEM_ERR_HIP);
+  }
+  s = launch_numbering(st, m, tabs, (int)nxg, (int)nyg, m->l2g, map->perm, map->seg);
+  hipStreamSynchronize(st);
+  hipFree(dtab);
+  if (s != SEM_OK) return fail(s);
   *out = m;
   return SEM_OK;
 }

@@ -79,6 +79,27 @@ def test_mesh_setup_matches_oracle(se, n, E, Ey, per, deform):
         np.testing.assert_array_equal(se.to_host(se.generateMask(bc, m)), o.generateMask(bc, mo))
 
 
+@pytest.mark.parametrize("nr,ns,E,Ey,per", [
+    (4, 6, 3, 2, (False, True)), (3, 5, 1, 4, (True, True)), (5, 3, 2, 7, (True, True)),
+    (2, 2, 1, 1, (True, True)), (6, 9, 4, 3, (True, False)),
+])
+def test_gather_scatter_map_rectangular_orders(se, nr, ns, E, Ey, per):
+    """The mesh's QQ' map (built on the device from the structured numbering)
+    for nr != ns and single-element periodic directions: value-exact against
+    the dense QQtx * u * QQty' and equal to the map built from l2g by the
+    host's counting sort (sem_gsmap_create)."""
+    _log(f"gs nr={nr} ns={ns} E={E}x{Ey} per={per}")
+    m = se.Mesh(nr, ns, E, Ey, per)
+    mo = o.Mesh(nr, ns, E, Ey, per)
+    np.testing.assert_array_equal(se.to_host(m.l2g).astype(np.int64), mo.l2g)
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(mo.x.shape)
+    ref = o.gatherScatter(u, mo.QQtx, mo.QQty)
+    got = se.to_host(se.gatherScatter(se.to_device(u), m))
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(got, np.asarray(se.gatherScatter(u, mo.l2g)))
+
+
 @pytest.mark.parametrize("n,E,Ey,per", [
     (8, 3, 2, (False, False)), (5, 4, 3, (True, False)), (4, 3, 3, (True, True)),
     (3, 1, 1, (True, True)), (9, 4, 4, (False, True)),
