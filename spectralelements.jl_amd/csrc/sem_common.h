@@ -79,6 +79,7 @@ struct sem_mesh {
   double* f[SEM_NUM_MESH_FIELDS] = {};
   double *Dr = nullptr, *Ds = nullptr, *zr = nullptr, *wr = nullptr, *zs = nullptr, *ws = nullptr;
   float *Dr32 = nullptr, *Ds32 = nullptr;
+  void* small = nullptr;   // the one device block Dr .. Ds32 point into
   std::vector<double> hDr, hDs, hzr, hwr, hzs, hws;
   int32_t* l2g = nullptr;
   sem_gsmap* map = nullptr;

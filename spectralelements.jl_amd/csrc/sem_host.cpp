@@ -217,7 +217,7 @@ void free_mesh(sem_mesh* m) {
   if (!m) return;
   for (int i = 0; i < SEM_NUM_MESH_FIELDS; ++i)
     if (m->f[i]) hipFree(m->f[i]);
-  void* ptrs[] = {m->Dr, m->Ds, m->zr, m->wr, m->zs, m->ws, m->Dr32, m->Ds32, m->l2g};
+  void* ptrs[] = {m->small, m->l2g};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (m->map) {
@@ -368,33 +368,58 @@ int sem_mesh_create(sem_ctx* ctx, int nr, int ns, int Ex, int Ey, int periodic_x
       return fail(SEM_ERR_HIP);
     }
   }
-  if ((s = upload(st, &m->Dr, m->hDr.data(), m->hDr.size())) != SEM_OK) return fail(s);
-  if ((s = upload(st, &m->Ds, m->hDs.data(), m->hDs.size())) != SEM_OK) return fail(s);
-  if ((s = upload(st, &m->zr, m->hzr.data(), nr)) != SEM_OK) return fail(s);
-  if ((s = upload(st, &m->wr, m->hwr.data(), nr)) != SEM_OK) return fail(s);
-  if ((s = upload(st, &m->zs, m->hzs.data(), ns)) != SEM_OK) return fail(s);
-  if ((s = upload(st, &m->ws, m->hws.data(), ns)) != SEM_OK) return fail(s);
-  {
-    std::vector<float> a(m->hDr.begin(), m->hDr.end()), b(m->hDs.begin(), m->hDs.end());
-    if ((s = upload(st, &m->Dr32, a.data(), a.size())) != SEM_OK) return fail(s);
-    if ((s = upload(st, &m->Ds32, b.data(), b.size())) != SEM_OK) return fail(s);
-  }
-  // coordinates (semmesh + ndgrid + deform) and FP64 geometry on the GPU
+  // the small device arrays (Dr, Ds, the GLL nodes and weights, FP32 Dr, Ds
+  // and the 1-D node coordinates semmesh gives) in one block and one copy
   std::vector<double> xe(L), ye(NC);
   semmesh(Ex, nr, xe.data(), nullptr);
   semmesh(Ey, ns, ye.data(), nullptr);
-  double *dxe = nullptr, *dye = nullptr;
-  if ((s = upload(st, &dxe, xe.data(), xe.size())) != SEM_OK) return fail(s);
-  if ((s = upload(st, &dye, ye.data(), ye.size())) != SEM_OK) {
-    hipFree(dxe);
-    return fail(s);
+  const std::vector<float> dr32(m->hDr.begin(), m->hDr.end()), ds32(m->hDs.begin(), m->hDs.end());
+  struct Piece {
+    const void* src;
+    size_t bytes;
+    size_t off;
+  };
+  Piece pcs[] = {{m->hDr.data(), sizeof(double) * m->hDr.size(), 0},
+                 {m->hDs.data(), sizeof(double) * m->hDs.size(), 0},
+                 {m->hzr.data(), sizeof(double) * nr, 0},
+                 {m->hwr.data(), sizeof(double) * nr, 0},
+                 {m->hzs.data(), sizeof(double) * ns, 0},
+                 {m->hws.data(), sizeof(double) * ns, 0},
+                 {dr32.data(), sizeof(float) * dr32.size(), 0},
+                 {ds32.data(), sizeof(float) * ds32.size(), 0},
+                 {xe.data(), sizeof(double) * xe.size(), 0},
+                 {ye.data(), sizeof(double) * ye.size(), 0}};
+  size_t total = 0;
+  for (Piece& pc : pcs) {
+    pc.off = total;
+    total += (pc.bytes + 255) / 256 * 256;
   }
+  std::vector<unsigned char> stage(total, 0);
+  for (const Piece& pc : pcs) std::memcpy(stage.data() + pc.off, pc.src, pc.bytes);
+  if (hipMalloc(&m->small, total) != hipSuccess) {
+    set_error("sem_mesh_create: hipMalloc of %zu bytes failed", total);
+    return fail(SEM_ERR_HIP);
+  }
+  if (hipMemcpyAsync(m->small, stage.data(), total, hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("sem_mesh_create: copying the node tables failed");
+    return fail(SEM_ERR_HIP);
+  }
+  unsigned char* blk = static_cast<unsigned char*>(m->small);
+  m->Dr = reinterpret_cast<double*>(blk + pcs[0].off);
+  m->Ds = reinterpret_cast<double*>(blk + pcs[1].off);
+  m->zr = reinterpret_cast<double*>(blk + pcs[2].off);
+  m->wr = reinterpret_cast<double*>(blk + pcs[3].off);
+  m->zs = reinterpret_cast<double*>(blk + pcs[4].off);
+  m->ws = reinterpret_cast<double*>(blk + pcs[5].off);
+  m->Dr32 = reinterpret_cast<float*>(blk + pcs[6].off);
+  m->Ds32 = reinterpret_cast<float*>(blk + pcs[7].off);
+  const double* dxe = reinterpret_cast<const double*>(blk + pcs[8].off);
+  const double* dye = reinterpret_cast<const double*>(blk + pcs[9].off);
+  // coordinates (semmesh + ndgrid + deform) and FP64 geometry on the GPU
   s = launch_coords(st, m, dxe, dye, deform_kind, deform_param);
   if (s == SEM_OK) s = launch_geometry(st, m);
   if (s == SEM_OK) s = launch_mult(st, m);
-  hipStreamSynchronize(st);
-  hipFree(dxe);
-  hipFree(dye);
+  hipStreamSynchronize(st);   // also keeps stage alive until the copy is done
   if (s != SEM_OK) return fail(s);
   // local-to-global numbering (semq.jl:6-25 in both directions, mesh.jl:87-89)
   // and the QQ' map, built on the device from per-direction tables: per
