@@ -530,6 +530,10 @@ struct sem_mg {
     // the level's applies carry the smoother's vector updates as a fused
     // epilogue (FP64 / FP32 cycle)
     bool fuse64 = false, fuse32 = false;
+    // the device blocks the level vectors above live in (one allocation per
+    // precision)
+    double* blk64 = nullptr;
+    float* blk32 = nullptr;
   };
   sem_mg_params prm{};
   int device = 0;
@@ -560,14 +564,10 @@ void mg_free(sem_mg* mg) {
   if (!mg) return;
   (void)hipDeviceSynchronize();
   for (auto& L : mg->lv) {
-    double* bufs[] = {L.dinv, L.J, L.r, L.d, L.w, L.e};
+    double* bufs[] = {L.blk64, L.J};
     for (double* p : bufs)
       if (p) (void)hipFree(p);
-    if (L.own_mesh) {
-      if (L.x) (void)hipFree(L.x);
-      if (L.b) (void)hipFree(L.b);
-    }
-    float* fbufs[] = {L.fx, L.fb, L.fr, L.fd, L.fw, L.fe, L.fdinv, L.fJ};
+    float* fbufs[] = {L.blk32, L.fJ};
     for (float* p : fbufs)
       if (p) (void)hipFree(p);
     if (L.op32) sem_operator_destroy(const_cast<sem_operator*>(L.op32));
@@ -808,16 +808,21 @@ int estimate_lmax(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& L) {
   return SEM_OK;
 }
 
+// the level's vectors (x and b only below the finest level) in one block,
+// each starting on a 512-byte boundary
 int level_buffers(sem_mg::Level& L, bool own_xb) {
   const long n = (long)L.mesh->nl;
-  SEM_TRY(dalloc(&L.dinv, n));
-  SEM_TRY(dalloc(&L.r, n));
-  SEM_TRY(dalloc(&L.d, n));
-  SEM_TRY(dalloc(&L.w, n));
-  SEM_TRY(dalloc(&L.e, n));
+  const long sz = (n + 63) / 64 * 64;
+  SEM_TRY(dalloc(&L.blk64, sz * (own_xb ? 7 : 5)));
+  double* p = L.blk64;
+  L.dinv = p;
+  L.r = p + sz;
+  L.d = p + 2 * sz;
+  L.w = p + 3 * sz;
+  L.e = p + 4 * sz;
   if (own_xb) {
-    SEM_TRY(dalloc(&L.x, n));
-    SEM_TRY(dalloc(&L.b, n));
+    L.x = p + 5 * sz;
+    L.b = p + 6 * sz;
   }
   return SEM_OK;
 }
@@ -1028,8 +1033,10 @@ int build(sem_ctx* ctx, sem_mg* mg) {
       SEM_TRY(sem_operator_create(ctx, L.mesh, &d, SEM_F32, &o32));
       L.op32 = o32;
       L.fuse32 = mg->fuse && operator_epi_ok(o32);
+      const long sz = (n + 127) / 128 * 128;
       float** fb[] = {&L.fx, &L.fb, &L.fr, &L.fd, &L.fw, &L.fe, &L.fdinv};
-      for (float** q : fb) SEM_TRY(dalloc(q, n));
+      SEM_TRY(dalloc(&L.blk32, sz * 7));
+      for (int q = 0; q < 7; ++q) *fb[q] = L.blk32 + q * sz;
       SEM_TRY(launch_convert_f64_f32(st, n, L.dinv, L.fdinv));
       if (L.nJ > 0) {
         SEM_TRY(dalloc(&L.fJ, L.nJ));
