@@ -119,12 +119,14 @@ struct AxP {
   // leaves x alone)
   T* __restrict__ x;
   PcgDev pcg;
-  // fused epilogue (EPI 1, 2; see ApplyEpi): er, ex and y are written, u is
+  // fused epilogue (EPI 1 to 3; see ApplyEpi): er, ex, ed and y are written, u is
   // only read, and none of them alias
   T* er;
   T* ex;
   const T* edinv;
   T ec1, ec2;
+  T* ed;       // EPI 3: the direction written by the smoother's start
+  T etheta;
 };
 
 // ---- MFMA contractions for 16x16 element blocks (MF != 0) ----------------
@@ -789,7 +791,8 @@ __global__ void __launch_bounds__(CX* P)
       auto fin = [&](int idx, int j, T val, T uin, int eb) {
         const T wv = val * mval<T>(i, j, L, NC, bc);
         if constexpr (EPI == 0) st_out(&prm.y[idx], wv);
-        else epi_store<T, EPI>(idx, wv, uin, ein[eb], prm.er, prm.ex, prm.ec1, prm.ec2, prm.y);
+        else epi_store<T, EPI>(idx, wv, uin, ein[eb], prm.er, prm.ex, prm.ec1, prm.ec2, prm.y,
+                               prm.ed, prm.etheta);
       };
 #pragma unroll
       for (int b = 0; b < P; ++b) {
@@ -862,7 +865,7 @@ constexpr int kSeamBlock = 128;
 template <typename T, int EPI = 0>
 __global__ void __launch_bounds__(kSeamBlock)
 k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int per_x, int per_y,
-        int bc, int nys, const T* u, T* er, T* ex, const T* edinv, T ec1, T ec2) {
+        int bc, int nys, const T* u, T* er, T* ex, const T* edinv, T ec1, T ec2, T* ed, T etheta) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   // a copy's epilogue operands (and input value) are loaded together with
   // the seam copies, then the update is stored from the final value
@@ -890,7 +893,7 @@ k_seams(T* y, int L, int NC, int P, int CXP, int nstrips, int SP, int nsegs, int
       if constexpr (SEM_SEAMS_NT) __builtin_nontemporal_store(wv, &y[c.t]);
       else y[c.t] = wv;
     }
-    else epi_store<T, EPI>(c.t, wv, c.u, c.e, er, ex, ec1, ec2, y);
+    else epi_store<T, EPI>(c.t, wv, c.u, c.e, er, ex, ec1, ec2, y, ed, etheta);
   };
   if ((int)blockIdx.y < nys) {
     const int g = blockIdx.y;
@@ -1004,13 +1007,14 @@ void launch_k1(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int mod
   else if (mode == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, MF>), grid, block, 0, st, prm);
   else hipLaunchKernelGGL((k_ax<T, P, CX, 0, MF>), grid, block, 0, st, prm);
 }
-// the two-row apply with a fused epilogue (kind 1 or 2): P = 2, 4, 8, the
+// the two-row apply with a fused epilogue (kind 1 to 3): P = 2, 4, 8, the
 // multigrid levels of N = 7 and 15
 template <typename T, int P, int CX>
 void launch_k1_epi(hipStream_t st, dim3 grid, dim3 block, const AxP<T>& prm, int kind) {
   if constexpr (CX % 2 == 0 && (P == 2 || P == 4 || P == 8)) {
     if (kind == 1) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2, 1>), grid, block, 0, st, prm);
     if (kind == 2) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2, 2>), grid, block, 0, st, prm);
+    if (kind == 3) hipLaunchKernelGGL((k_ax<T, P, CX, 1, 0, 2, 3>), grid, block, 0, st, prm);
   }
 }
 template <typename T, int P, int CX>
@@ -1052,8 +1056,9 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
   if (mf == 0 && P == 16 && CX * P == 256 && !a.fma16) mf = sizeof(T) == 8 ? 6 : 5;
   const int epi = a.epi.kind;
   if (epi != 0) {
-    if (!(epi == 1 || epi == 2) || mf != 0 || mode != 1 || pl.RY != 2 ||
-        !(P == 2 || P == 4 || P == 8) || !a.epi.r || (epi == 1 && (!a.epi.x || !a.epi.dinv))) {
+    if (!(epi >= 1 && epi <= 3) || mf != 0 || mode != 1 || pl.RY != 2 ||
+        !(P == 2 || P == 4 || P == 8) || !a.epi.r ||
+        (epi != 2 && (!a.epi.x || !a.epi.dinv)) || (epi == 3 && !a.epi.d)) {
       set_error("fused apply epilogue %d: unsupported launch", epi);
       return SEM_ERR_UNSUPPORTED;
     }
@@ -1062,6 +1067,8 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
     prm.edinv = static_cast<const T*>(a.epi.dinv);
     prm.ec1 = T(a.epi.c1);
     prm.ec2 = T(a.epi.c2);
+    prm.ed = static_cast<T*>(a.epi.d);
+    prm.etheta = T(a.epi.theta);
     launch_k1_epi<T, P, CX>(st, grid, block, prm, epi);
   } else if (mf == 0 && pl.RY > 1) {
     launch_k1_rows<T, P, CX>(st, grid, block, prm, pl.RY, mode);
@@ -1100,15 +1107,19 @@ int launch_apply_p(hipStream_t st, const Layout& g, const ApplyArgs& a, const Pc
     if (epi == 1)
       hipLaunchKernelGGL((k_seams<T, 1>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
                          pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
-                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2);
+                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2, prm.ed, prm.etheta);
     else if (epi == 2)
       hipLaunchKernelGGL((k_seams<T, 2>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
                          pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
-                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2);
+                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2, prm.ed, prm.etheta);
+    else if (epi == 3)
+      hipLaunchKernelGGL((k_seams<T, 3>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
+                         pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
+                         prm.u, prm.er, prm.ex, prm.edinv, prm.ec1, prm.ec2, prm.ed, prm.etheta);
     else
       hipLaunchKernelGGL((k_seams<T, 0>), sgrid, dim3(kSeamBlock), 0, st, prm.y, g.L, g.NC, P,
                          pl.CX * P, pl.nstrips, pl.S * P, pl.nsegs, g.per_x, g.per_y, a.bc, nys,
-                         nullptr, nullptr, nullptr, nullptr, T(0), T(0));
+                         nullptr, nullptr, nullptr, nullptr, T(0), T(0), nullptr, T(1));
   }
   SEM_HIP(hipGetLastError());
   return SEM_OK;

@@ -644,14 +644,26 @@ int apply(sem_ctx* ctx, const sem_operator* op, const void* u, void* y) {
 }
 
 // r -= A v (fused into the apply's final writes where the level allows;
-// w is scratch either way)
+// w is scratch either way); with start_theta the fused form also starts the
+// Chebyshev smoother from the new r (d = dinv .* r / theta; x += d) and
+// returns *started = true
 template <typename T>
-int residual_update(sem_ctx* ctx, const LvBuf<T>& L, const T* v) {
+int residual_update(sem_ctx* ctx, const LvBuf<T>& L, const T* v, const double* start_theta = nullptr,
+                    bool* started = nullptr) {
+  if (started) *started = false;
   if (L.fuse) {
     ApplyEpi e;
-    e.kind = 2;
+    e.kind = start_theta ? 3 : 2;
     e.r = L.r;
-    return operator_apply_epi(ctx, L.op, v, L.w, e);
+    if (start_theta) {
+      e.x = L.x;
+      e.dinv = L.dinv;
+      e.d = L.d;
+      e.theta = *start_theta;
+    }
+    SEM_TRY(operator_apply_epi(ctx, L.op, v, L.w, e));
+    if (started) *started = start_theta != nullptr;
+    return SEM_OK;
   }
   SEM_TRY(apply(ctx, L.op, v, L.w));
   const long n = (long)L.op->mesh->nl;
@@ -665,7 +677,7 @@ int residual_update(sem_ctx* ctx, const LvBuf<T>& L, const T* v) {
 // x is formed in r (one more apply).
 template <typename T>
 int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& Lv, const T* b, int from_zero, int deg,
-          bool want_r) {
+          bool want_r, bool started = false) {
   hipStream_t st = ctx->stream;
   LvBuf<T> L = bufs<T>(Lv);
   const long n = (long)Lv.mesh->nl;
@@ -673,9 +685,11 @@ int cheby(sem_ctx* ctx, sem_mg* mg, sem_mg::Level& Lv, const T* b, int from_zero
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
-  hipLaunchKernelGGL((k_cheb_start<T>), dim3(nblk(n)), dim3(256), 0, st, b, L.r, L.dinv, L.d,
-                     L.x, theta, from_zero, n);
-  SEM_HIP(hipGetLastError());
+  if (!started) {   // else the residual update formed d and advanced x
+    hipLaunchKernelGGL((k_cheb_start<T>), dim3(nblk(n)), dim3(256), 0, st, b, L.r, L.dinv, L.d,
+                       L.x, theta, from_zero, n);
+    SEM_HIP(hipGetLastError());
+  }
   for (int i = 1; i < deg; ++i) {
     const double rho_n = 1.0 / (2.0 * sigma - rho);
     if (L.fuse) {
@@ -761,8 +775,11 @@ int vcycle(sem_ctx* ctx, sem_mg* mg, int l) {
                               C.b));
   SEM_TRY(vcycle<T>(ctx, mg, l + 1));
   SEM_TRY(launch_prolong<T>(st, C.x, L.J, L.e, L.x, Lv.xf));
-  SEM_TRY(residual_update<T>(ctx, L, L.e));
-  return cheby<T>(ctx, mg, Lv, L.b, 0, mg->prm.degree, false);
+  // the post-smoother's start (from r) rides on the residual update
+  const double theta = 0.5 * (Lv.lmax + Lv.lmax / mg->prm.ratio);
+  bool started = false;
+  SEM_TRY(residual_update<T>(ctx, L, L.e, &theta, &started));
+  return cheby<T>(ctx, mg, Lv, L.b, 0, mg->prm.degree, false, started);
 }
 
 // lmax = safety x Rayleigh quotient after npow power steps of D^-1 A

@@ -143,14 +143,18 @@ int launch_mask_fill(hipStream_t st, const Layout& g, int bits, double* M);
 // Elementwise epilogue fused into an assembled apply (multigrid smoothing),
 // applied to each node copy where its assembled, masked value w = (A u)_i is
 // final: kind 1 (a Chebyshev step, u = d): r -= w; d' = c1 d + c2 dinv .* r;
-// x += d', with d' stored in y; kind 2: r -= w (y is then scratch). kind 0:
-// a plain apply.
+// x += d', with d' stored in y; kind 2: r -= w (y is then scratch); kind 3
+// (a residual update that starts the Chebyshev smoother): r -= w;
+// d = dinv .* r / theta; x += d, with d a separate buffer (y is scratch).
+// kind 0: a plain apply.
 struct ApplyEpi {
   int kind = 0;
   void* r = nullptr;
   void* x = nullptr;
   const void* dinv = nullptr;
   double c1 = 0.0, c2 = 0.0;
+  void* d = nullptr;
+  double theta = 1.0;
 };
 
 struct ApplyArgs {

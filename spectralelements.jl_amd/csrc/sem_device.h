@@ -219,7 +219,7 @@ template <typename T, int KIND>
 __device__ __forceinline__ EpiIn<T> epi_load(long t, const T* r, const T* x, const T* dinv) {
   EpiIn<T> v{};
   v.r = r[t];
-  if constexpr (KIND == 1) {
+  if constexpr (KIND == 1 || KIND == 3) {
     v.dinv = dinv[t];
     v.x = x[t];
   }
@@ -227,12 +227,16 @@ __device__ __forceinline__ EpiIn<T> epi_load(long t, const T* r, const T* x, con
 }
 template <typename T, int KIND>
 __device__ __forceinline__ void epi_store(long t, T w, T dold, const EpiIn<T>& v, T* r, T* x,
-                                          T c1, T c2, T* y) {
+                                          T c1, T c2, T* y, T* d, T theta) {
   const T rv = v.r - w;
   r[t] = rv;
   if constexpr (KIND == 1) {
     const T dv = cheb_dir(c1, dold, c2, v.dinv, rv);
     y[t] = dv;
+    x[t] = v.x + dv;
+  } else if constexpr (KIND == 3) {
+    const T dv = v.dinv * rv / theta;   // the Chebyshev start's expression
+    d[t] = dv;
     x[t] = v.x + dv;
   }
 }
