@@ -479,14 +479,28 @@ __global__ void k_times_diag(const double* __restrict__ v, const double* __restr
   }
 }
 
-// x += alpha p, r -= alpha w
+// x += alpha p, r -= alpha w with alpha = <z, r> / <p, A p> read from the
+// device (pcg.jl:50-52)
 __global__ void k_xr(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
-                     const double* __restrict__ w, double alpha, long n) {
+                     const double* __restrict__ w, const double* __restrict__ tz,
+                     const double* __restrict__ tpw, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double alpha = *tz / *tpw;
   if (t < n) {
     x[t] = x[t] + alpha * p[t];
     r[t] = r[t] - alpha * w[t];
   }
+}
+
+// p = a z + beta p with beta = t / t_prev, or (t - s) / t_prev when s is
+// given (the flexible form of the FP32 cycle), read from the device
+// (pcg.jl:49)
+__global__ void k_pdir(double a, const double* __restrict__ z, double* p,
+                       const double* __restrict__ tz, const double* __restrict__ tprev,
+                       const double* __restrict__ s, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double beta = (s ? *tz - *s : *tz) / *tprev;
+  if (t < n) p[t] = a * z[t] + beta * p[t];
 }
 
 // Interpolation matrix rows with exact unit rows where an output point equals
@@ -559,6 +573,9 @@ struct sem_mg {
 namespace {
 
 constexpr int kParts = 256;
+// device scalars after the partials: [0] reduction result, [1], [2] <z, r> of
+// the even and odd PCG iterations, [3] <p, A p>, [4] the flexible beta's s
+constexpr int kScalars = 8;
 
 void mg_free(sem_mg* mg) {
   if (!mg) return;
@@ -1129,7 +1146,7 @@ int sem_mg_create(sem_ctx* ctx, const sem_operator* op, const sem_mg_params* par
   mg->fuse = !(fz && std::atoi(fz) == 0);
   const char* fb = std::getenv("SEM_MG_BLOCK");
   mg->block = !(fb && std::atoi(fb) == 0);
-  if (dalloc(&mg->part, kParts + 1) != SEM_OK) {
+  if (dalloc(&mg->part, kParts + kScalars) != SEM_OK) {
     mg_free(mg);
     return SEM_ERR_HIP;
   }
@@ -1203,7 +1220,8 @@ int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b
   SEM_HIP(hipEventRecord(ctx->ev0, st));
   SEM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
   SEM_HIP(hipMemcpyAsync(mg->pr, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-  double res = 0.0, t = 0.0, t_prev = 1.0;
+  double res = 0.0;
+  double* sc = mg->part + kParts;
   SEM_TRY(norm_host(ctx, mg, n, mg->pr, &res));
   int k = 0;
   bool conv = true;
@@ -1214,30 +1232,29 @@ int sem_pcg_mg(sem_ctx* ctx, const sem_operator* op, sem_mg* mg, const double* b
     }
     ++k;
     SEM_TRY(sem_mg_vcycle(ctx, mg, mg->pr, mg->pz));
-    SEM_TRY(wdot_host(ctx, mg, g, mg->pr, mg->pz, &t));
+    // the scalars stay on the device: the loop waits once per iteration, for
+    // the residual norm of its convergence test
+    double* tz = sc + 1 + (k & 1);
+    double* tprev = sc + 1 + ((k - 1) & 1);
+    SEM_TRY(launch_wdot(st, g, mg->pr, mg->pz, mg->part, kParts, tz));
     if (k == 1) {
       SEM_HIP(hipMemcpyAsync(mg->pp, mg->pz, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     } else {
       // FP32 cycle: the flexible (Polak-Ribiere) beta = <z_k, r_k - r_{k-1}>
       // / t_prev, which tolerates the cycle's rounding; FP64: pcg.jl:49
-      double beta = t / t_prev;
-      if (mg->f32) {
-        double s = 0.0;
-        SEM_TRY(wdot_host(ctx, mg, g, mg->pq, mg->pz, &s));
-        beta = (t - s) / t_prev;
-      }
-      SEM_TRY(launch_axpby(st, n, 1.0, mg->pz, beta, mg->pp, mg->pp));
+      if (mg->f32) SEM_TRY(launch_wdot(st, g, mg->pq, mg->pz, mg->part, kParts, sc + 4));
+      hipLaunchKernelGGL(k_pdir, dim3(nblk(n)), dim3(256), 0, st, 1.0, mg->pz, mg->pp, tz, tprev,
+                         mg->f32 ? sc + 4 : nullptr, n);
+      SEM_HIP(hipGetLastError());
     }
     if (mg->f32)
       SEM_HIP(hipMemcpyAsync(mg->pq, mg->pr, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     SEM_TRY(apply(ctx, op, mg->pp, mg->pw));
-    double pw = 0.0;
-    SEM_TRY(wdot_host(ctx, mg, g, mg->pp, mg->pw, &pw));
-    const double alpha = t / pw;
-    hipLaunchKernelGGL(k_xr, dim3(nblk(n)), dim3(256), 0, st, x, mg->pr, mg->pp, mg->pw, alpha, n);
+    SEM_TRY(launch_wdot(st, g, mg->pp, mg->pw, mg->part, kParts, sc + 3));
+    hipLaunchKernelGGL(k_xr, dim3(nblk(n)), dim3(256), 0, st, x, mg->pr, mg->pp, mg->pw, tz,
+                       sc + 3, n);
     SEM_HIP(hipGetLastError());
     SEM_TRY(norm_host(ctx, mg, n, mg->pr, &res));
-    t_prev = t;
     if (!std::isfinite(res)) break;
   }
   SEM_HIP(hipEventRecord(ctx->ev1, st));
