@@ -236,15 +236,28 @@ k_wdot_part(const double* __restrict__ a, const double* __restrict__ b, int P, i
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// max that keeps a NaN once seen (from either side)
+__device__ __forceinline__ double nan_max(double m, double v) {
+  return (v > m || v != v) ? v : m;
+}
+
+// four independent loads in flight per step (the max is exact and
+// order-free, so the result does not depend on the grouping)
 __global__ void __launch_bounds__(256)
 k_absmax_part(const double* __restrict__ a, long n, double* part) {
   __shared__ double sh[256];
-  double m = 0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    const double v = fabs(a[t]);
-    m = (v > m || v != v) ? v : m;
+  const long stride = (long)gridDim.x * blockDim.x;
+  double m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t + 3 * stride < n; t += 4 * stride) {
+    const double v0 = a[t], v1 = a[t + stride], v2 = a[t + 2 * stride], v3 = a[t + 3 * stride];
+    m0 = nan_max(m0, fabs(v0));
+    m1 = nan_max(m1, fabs(v1));
+    m2 = nan_max(m2, fabs(v2));
+    m3 = nan_max(m3, fabs(v3));
   }
+  for (; t < n; t += stride) m0 = nan_max(m0, fabs(a[t]));
+  const double m = nan_max(nan_max(m0, m1), nan_max(m2, m3));
   const double r = block_reduce_max(m, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
