@@ -202,33 +202,38 @@ __device__ double block_reduce_max(double v, double* sh) {
 }
 
 // Block partials of sum(a .* b .* mult): block q takes columns
-// [q cpb, (q + 1) cpb) whole (the column's multiplicity is uniform), its
-// threads the rows, four rows per pass with their loads issued together (one
-// block per CU keeps few waves resident), so there is no 64-bit division per
-// node and enough loads in flight.
-__global__ void __launch_bounds__(256)
+// [q cpb, (q + 1) cpb) whole, one contiguous run of nodes; its 1,024 threads
+// step through the run four nodes each per pass with their loads issued
+// together (one block per CU: sixteen waves with eight loads each in
+// flight). Node f of the run lies in column j0 + f / L, row f % L (32-bit
+// arithmetic: a run holds fewer than 2^31 nodes).
+constexpr int kWdotThreads = 1024;
+__global__ void __launch_bounds__(kWdotThreads)
 k_wdot_part(const double* __restrict__ a, const double* __restrict__ b, int P, int L, int NC,
             int per_x, int per_y, double* part) {
-  __shared__ double sh[256];
+  __shared__ double sh[kWdotThreads];
   const int cpb = (NC + (int)gridDim.x - 1) / (int)gridDim.x;
   const int j0 = blockIdx.x * cpb, j1 = min(NC, j0 + cpb);
+  const int cnt = j1 > j0 ? (j1 - j0) * L : 0;
+  const double* __restrict__ ab = a + (long)L * j0;
+  const double* __restrict__ bb = b + (long)L * j0;
   const int bs = blockDim.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = j0; j < j1; ++j) {
-    const double mj = mult1(j, P, NC, per_y);
-    const long cj = (long)L * j;
-    for (int i0 = threadIdx.x; i0 < L; i0 += 4 * bs) {
-      double av[4], bv[4];
+  for (int f0 = threadIdx.x; f0 < cnt; f0 += 4 * bs) {
+    double av[4], bv[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = min(i0 + u * bs, L - 1);
-        av[u] = a[cj + i];
-        bv[u] = b[cj + i];
-      }
+    for (int u = 0; u < 4; ++u) {
+      const int f = min(f0 + u * bs, cnt - 1);
+      av[u] = ab[f];
+      bv[u] = bb[f];
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * bs;
-        if (i < L) acc[u] += av[u] * bv[u] * (mult1(i, P, L, per_x) * mj);
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * bs;
+      if (f < cnt) {
+        const int jj = f / L;
+        const int i = f - jj * L;
+        acc[u] += av[u] * bv[u] * (mult1(i, P, L, per_x) * mult1(j0 + jj, P, NC, per_y));
       }
     }
   }
@@ -825,8 +830,13 @@ int launch_abu(hipStream_t st, int m, int n, const double* u, const double* As, 
 
 int launch_wdot(hipStream_t st, const Layout& g, const double* a, const double* b, double* part,
                 int nparts, double* out) {
-  hipLaunchKernelGGL(k_wdot_part, dim3(nparts), dim3(256), 0, st, a, b, g.P, g.L, g.NC, g.per_x,
-                     g.per_y, part);
+  if ((int64_t)((g.NC + nparts - 1) / nparts) * g.L > INT32_MAX) {
+    set_error("weighted dot: %d columns of %d rows in %d parts exceed 2^31 nodes per part", g.NC,
+              g.L, nparts);
+    return SEM_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(k_wdot_part, dim3(nparts), dim3(kWdotThreads), 0, st, a, b, g.P, g.L, g.NC,
+                     g.per_x, g.per_y, part);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
