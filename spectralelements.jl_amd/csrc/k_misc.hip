@@ -303,15 +303,31 @@ __global__ void k_combine(CombineTerms terms, const double* __restrict__ B, doub
   y[t] = acc;
 }
 
+// four strided nodes per step, their loads issued together, each summed
+// into its own accumulator; HB, HW: b, w given (a branch-free loop body)
+template <bool HB, bool HW>
 __global__ void __launch_bounds__(256)
 k_dot_part(const double* __restrict__ a, const double* __restrict__ b,
            const double* __restrict__ w, long n, double* part) {
   __shared__ double sh[256];
-  double acc = 0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x)
-    acc += b ? (w ? a[t] * b[t] * w[t] : a[t] * b[t]) : (w ? a[t] * w[t] : a[t]);
-  const double s = block_reduce_sum(acc, sh);
+  auto term = [&](long i) {
+    if constexpr (HB && HW) return a[i] * b[i] * w[i];
+    else if constexpr (HB) return a[i] * b[i];
+    else if constexpr (HW) return a[i] * w[i];
+    else return a[i];
+  };
+  const long stride = (long)gridDim.x * blockDim.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t + 3 * stride < n; t += 4 * stride) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = term(t + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] += v[u];
+  }
+  for (; t < n; t += stride) acc[0] += term(t);
+  const double s = block_reduce_sum((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -851,9 +867,27 @@ int launch_combine(hipStream_t st, const sem_mesh* m, const CombineTerms& terms,
   return SEM_OK;
 }
 
+static int launch_dot_part(hipStream_t st, int64_t n, const double* a, const double* b,
+                           const double* w, double* part, int nparts) {
+  if (b && w)
+    hipLaunchKernelGGL((k_dot_part<true, true>), dim3(nparts), dim3(256), 0, st, a, b, w, (long)n,
+                       part);
+  else if (b)
+    hipLaunchKernelGGL((k_dot_part<true, false>), dim3(nparts), dim3(256), 0, st, a, b, w,
+                       (long)n, part);
+  else if (w)
+    hipLaunchKernelGGL((k_dot_part<false, true>), dim3(nparts), dim3(256), 0, st, a, b, w,
+                       (long)n, part);
+  else
+    hipLaunchKernelGGL((k_dot_part<false, false>), dim3(nparts), dim3(256), 0, st, a, b, w,
+                       (long)n, part);
+  SEM_HIP(hipGetLastError());
+  return SEM_OK;
+}
+
 int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
                double* part, int nparts, double* out) {
-  hipLaunchKernelGGL(k_dot_part, dim3(nparts), dim3(256), 0, st, a, b, w, (long)n, part);
+  SEM_TRY(launch_dot_part(st, n, a, b, w, part, nparts));
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 0, out);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
@@ -861,7 +895,7 @@ int launch_dot(hipStream_t st, int64_t n, const double* a, const double* b, cons
 
 int launch_dot_acc(hipStream_t st, int64_t n, const double* a, const double* b, const double* w,
                    double scale, double* part, int nparts, double* acc) {
-  hipLaunchKernelGGL(k_dot_part, dim3(nparts), dim3(256), 0, st, a, b, w, (long)n, part);
+  SEM_TRY(launch_dot_part(st, n, a, b, w, part, nparts));
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, st, part, nparts, 2, acc, scale);
   SEM_HIP(hipGetLastError());
   return SEM_OK;
